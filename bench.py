@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Headline benchmark: BabyBear batched coset LDE, 2^20 rows x 256 columns, 2x
+blowup (2^21 rows out), on one MI355X.
+
+A step is one LDE over the whole batch (iDFT -> n^-1 * shift^j -> zero-pad ->
+DFT) with the input already resident in HBM.  `value` is output elements per
+second (Melem/s), the reference's LDE throughput unit (benches/ntt_benchmark.rs:516-517).
+The default workload is the configuration as stated (row-major batch layout,
+shift = field generator 31); --semantics reference runs the reference's own
+SoA layout with shift = omega_{2N} (src/lde/mod.rs:59).  Both produce
+bit-exact results checked by the tests.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "small-field-metal-ntt_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "lde_2e20x256_hbm_counters.txt")
+
+
+def kernel_bytes(label: str, log_n: int, batch: int, added_bits: int) -> int:
+    """Algorithmic bytes of one launch: each value it must read once + each it writes once."""
+    n, N = batch << log_n, batch << (log_n + added_bits)
+    if label.startswith("k_lde_mid"):
+        return 4 * (n + N)
+    if "inv" in label:  # iDFT low layers: n in, n out
+        return 8 * n
+    return 8 * N        # DFT low layers over the extended columns
+
+
+def read_traffic(label_prefix: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    if not os.path.exists(COUNTERS_FILE):
+        return None
+    for line in open(COUNTERS_FILE):
+        parts = line.split()
+        if parts and parts[0] == label_prefix and len(parts) >= 2:
+            kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
+            if "hbm_bytes" in kv:
+                return float(kv["hbm_bytes"])
+    return None
+
+
+def cpu_baseline(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw: int,
+                 gpu_first_cols=None) -> dict:
+    """The oracle's restatement of coset_lde_batch_cpu (src/lde/mod.rs:127-176),
+    one thread, on the first `cols` columns of the same input."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    orc.lib()
+    t0 = time.perf_counter()
+    out = orc.coset_lde_batch(x_soa_sample, log_n, cols, added_bits, shift_raw, threads=1)
+    dt = time.perf_counter() - t0
+    res = {"value": round(cols * (1 << (log_n + added_bits)) / dt / 1e6, 3), "unit": "Melem/s",
+           "cores": 1, "kind": "port",
+           "sample": f"{cols} of 256 columns, 2^{log_n} rows, {1 << added_bits}x, SoA, {dt:.2f} s, "
+                     f"C restatement of coset_lde_batch_cpu, 1 thread of {os.cpu_count()}"}
+    if gpu_first_cols is not None:
+        res["parity_sample"] = bool(np.array_equal(out, gpu_first_cols))
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--added-bits", type=int, default=1)
+    ap.add_argument("--semantics", choices=["as-stated", "reference"], default="as-stated")
+    ap.add_argument("--cpu-cols", type=int, default=24, help="columns in the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not offered "
+              "(the LDE shards by column with no exchange, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import sfntt
+    from sfntt import synth
+
+    log_n, B, a = args.log_n, args.batch, args.added_bits
+    n, N = 1 << log_n, 1 << (log_n + a)
+    row_major = args.semantics == "as-stated"
+    layout = sfntt.LAYOUT_ROW_MAJOR if row_major else sfntt.LAYOUT_SOA
+    shift_raw = (31 << 32) % synth.BB_P if row_major else 0
+
+    ctx = sfntt.HipContext(0)
+    lib = ctx._lib
+    info = ctx.device_info()
+    # reference generation order: column-major stream, seed 42 + log_n (benches/ntt_benchmark.rs:482-490)
+    x = synth.field_data(1, 42 + log_n, n * B)
+    x_dev = x.reshape(B, n).T.copy().ravel() if row_major else x
+    d_in = sfntt.DeviceBuffer(ctx, n * B)
+    d_out = sfntt.DeviceBuffer(ctx, N * B)
+    d_in.upload(x_dev)
+
+    def step():
+        ctx.check(lib.sfn_coset_lde_batch_device(ctx.ptr, log_n, B, a, shift_raw, layout,
+                                                 d_in.ptr, d_out.ptr, None))
+
+    for _ in range(args.warmup):
+        step()
+    ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = N * B * args.steps / elapsed / 1e6
+
+    # per-kernel durations with hipEvent pairs on the launch stream
+    lib.sfn_pass_timing(ctx.ptr, 1)
+    per = {}
+    seq_ns = []
+    buf = (ctypes.c_uint64 * 16)()
+    cnt = ctypes.c_uint32()
+    for _ in range(args.steps):
+        step()
+        ctx.check(lib.sfn_pass_times(ctx.ptr, buf, 16, ctypes.byref(cnt)))
+        tot = 0
+        for i in range(cnt.value):
+            lab = lib.sfn_pass_label(ctx.ptr, i).decode()
+            per.setdefault(lab, []).append(buf[i])
+            tot += buf[i]
+        seq_ns.append(tot)
+    lib.sfn_pass_timing(ctx.ptr, 0)
+    kernels = {k: sum(v) / len(v) / 1e3 for k, v in per.items()}  # us
+    dom = max(kernels, key=kernels.get)
+    dom_bytes = kernel_bytes(dom, log_n, B, a)
+    achieved = dom_bytes / (kernels[dom] * 1e-6) / 1e9
+    traffic = read_traffic(dom.split("<")[0])
+    lde_bytes = 4 * B * (n + N)
+    pipe_gbs = lde_bytes / (ms_per_step * 1e-3) / 1e9
+
+    out_first = None
+    cb = None
+    if args.cpu_cols > 0:
+        cols = min(args.cpu_cols, B)
+        y = d_out.download()
+        out_first = (y.reshape(N, B).T[:cols].ravel() if row_major else y[:cols * N])
+        cb = cpu_baseline(x[:cols * n], log_n, cols, a, shift_raw, out_first)
+
+    line = {
+        "metric": "coset-LDE Melem/s (output elements) + % HBM roofline",
+        "value": round(value, 2),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (reference LCG, seed 42+log_n, BabyBear Montgomery form)",
+        "config": {
+            "workload": f"babybear_coset_lde_2^{log_n}x{B}_{1 << a}x",
+            "log_n": log_n, "batch": B, "added_bits": a,
+            "layout": "row_major" if row_major else "soa",
+            "shift": "field generator 31" if row_major else "two_adic_generator(log_n+added_bits+1)",
+            "device": info["name"] or info["gcn_arch"],
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "pipeline": {
+            "algorithmic_bytes": lde_bytes,
+            "achieved_GBs": round(pipe_gbs, 1),
+            "frac_hbm": round(pipe_gbs / HBM_PEAK_GBS, 4),
+            "device_ms_events": round(sum(seq_ns) / len(seq_ns) / 1e6, 4),
+            "kernels_us": {k: round(v, 1) for k, v in kernels.items()},
+        },
+        "cpu_baseline": cb,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

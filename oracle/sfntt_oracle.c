@@ -241,12 +241,7 @@ static int log2_exact(size_t n, uint32_t *log_n) {
 
 /* src/ntt/m31/cpu_reference.rs:46-76 (forward: layers log_n-1..0, CT butterfly
  * with the block-indexed twiddle, :134-139). */
-int orc_m31_forward(uint32_t *data, size_t n) {
-    uint32_t log_n;
-    if (log2_exact(n, &log_n)) return -1;
-    if (n == 1) return 0;
-    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_m31_twiddles(log_n, 0, tw);
+static void m31_forward_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (int layer = (int)log_n - 1; layer >= 0; --layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (n - ((size_t)1 << (log_n - layer)));
@@ -261,18 +256,22 @@ int orc_m31_forward(uint32_t *data, size_t n) {
             }
         }
     }
+}
+
+int orc_m31_forward(uint32_t *data, size_t n) {
+    uint32_t log_n;
+    if (log2_exact(n, &log_n)) return -1;
+    if (n == 1) return 0;
+    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
+    orc_m31_twiddles(log_n, 0, tw);
+    m31_forward_tw(data, n, log_n, tw);
     free(tw);
     return 0;
 }
 
 /* src/ntt/m31/cpu_reference.rs:78-114 (inverse: layers 0..log_n-1, GS butterfly
  * :145-150, then multiply by inv(n)). */
-int orc_m31_inverse(uint32_t *data, size_t n) {
-    uint32_t log_n;
-    if (log2_exact(n, &log_n)) return -1;
-    if (n == 1) return 0;
-    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_m31_twiddles(log_n, 1, tw);
+static void m31_inverse_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (uint32_t layer = 0; layer < log_n; ++layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (n - ((size_t)1 << (log_n - layer)));
@@ -288,19 +287,22 @@ int orc_m31_inverse(uint32_t *data, size_t n) {
     }
     uint32_t inv_n = orc_m31_inv(orc_m31_reduce((uint64_t)n));
     for (size_t i = 0; i < n; ++i) data[i] = orc_m31_mul(data[i], inv_n);
+}
+
+int orc_m31_inverse(uint32_t *data, size_t n) {
+    uint32_t log_n;
+    if (log2_exact(n, &log_n)) return -1;
+    if (n == 1) return 0;
+    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
+    orc_m31_twiddles(log_n, 1, tw);
+    m31_inverse_tw(data, n, log_n, tw);
     free(tw);
     return 0;
 }
 
 /* src/ntt/babybear/cpu_reference.rs:34-62 (forward_ntt_u32: GS-DIF, layers
  * log_n-1..0, position-indexed twiddle tw[layer][j]). */
-int orc_bb_forward(uint32_t *data, size_t n) {
-    uint32_t log_n;
-    if (log2_exact(n, &log_n)) return -1;
-    if (n == 1) return 0;
-    if (log_n > 27) return -1;
-    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_bb_twiddles(log_n, 0, tw);
+static void bb_forward_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (int layer = (int)log_n - 1; layer >= 0; --layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (half - 1);
@@ -313,19 +315,23 @@ int orc_bb_forward(uint32_t *data, size_t n) {
             }
         }
     }
+}
+
+int orc_bb_forward(uint32_t *data, size_t n) {
+    uint32_t log_n;
+    if (log2_exact(n, &log_n)) return -1;
+    if (n == 1) return 0;
+    if (log_n > 27) return -1;
+    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
+    orc_bb_twiddles(log_n, 0, tw);
+    bb_forward_tw(data, n, log_n, tw);
     free(tw);
     return 0;
 }
 
 /* src/ntt/babybear/cpu_reference.rs:69-103 (inverse_ntt_u32: CT-DIT, layers
  * 0..log_n-1, then multiply by inv(n)). */
-int orc_bb_inverse(uint32_t *data, size_t n) {
-    uint32_t log_n;
-    if (log2_exact(n, &log_n)) return -1;
-    if (n == 1) return 0;
-    if (log_n > 27) return -1;
-    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_bb_twiddles(log_n, 1, tw);
+static void bb_inverse_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (uint32_t layer = 0; layer < log_n; ++layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (half - 1);
@@ -341,6 +347,16 @@ int orc_bb_inverse(uint32_t *data, size_t n) {
     }
     uint32_t inv_n = orc_bb_inv(orc_bb_reduce((uint64_t)n));
     for (size_t i = 0; i < n; ++i) data[i] = orc_bb_mul(data[i], inv_n);
+}
+
+int orc_bb_inverse(uint32_t *data, size_t n) {
+    uint32_t log_n;
+    if (log2_exact(n, &log_n)) return -1;
+    if (n == 1) return 0;
+    if (log_n > 27) return -1;
+    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
+    orc_bb_twiddles(log_n, 1, tw);
+    bb_inverse_tw(data, n, log_n, tw);
     free(tw);
     return 0;
 }
@@ -350,7 +366,9 @@ int orc_bb_inverse(uint32_t *data, size_t n) {
 typedef struct {
     int field, dir;
     size_t n;
-    uint32_t *data; /* SoA */
+    uint32_t log_n;
+    const uint32_t *tw; /* table built once per call, like the reference's TwiddleCache */
+    uint32_t *data;     /* SoA */
     uint32_t c0, c1;
 } batch_job;
 
@@ -358,10 +376,11 @@ static void *batch_worker(void *arg) {
     batch_job *j = (batch_job *)arg;
     for (uint32_t c = j->c0; c < j->c1; ++c) {
         uint32_t *col = j->data + (size_t)c * j->n;
+        if (j->n == 1) continue;
         if (j->field == 0)
-            j->dir == 0 ? orc_m31_forward(col, j->n) : orc_m31_inverse(col, j->n);
+            j->dir == 0 ? m31_forward_tw(col, j->n, j->log_n, j->tw) : m31_inverse_tw(col, j->n, j->log_n, j->tw);
         else
-            j->dir == 0 ? orc_bb_forward(col, j->n) : orc_bb_inverse(col, j->n);
+            j->dir == 0 ? bb_forward_tw(col, j->n, j->log_n, j->tw) : bb_inverse_tw(col, j->n, j->log_n, j->tw);
     }
     return NULL;
 }
@@ -373,6 +392,13 @@ static void transpose(const uint32_t *src, uint32_t *dst, size_t rows, size_t co
 }
 
 static void run_columns(int field, int dir, size_t n, uint32_t batch, uint32_t *soa, int threads) {
+    uint32_t log_n = 0;
+    while (((size_t)1 << log_n) < n) ++log_n;
+    uint32_t *tw = (uint32_t *)malloc((n > 1 ? n - 1 : 1) * sizeof(uint32_t));
+    if (n > 1) {
+        if (field == 0) orc_m31_twiddles(log_n, dir, tw);
+        else orc_bb_twiddles(log_n, dir, tw);
+    }
     if (threads < 1) threads = 1;
     if ((uint32_t)threads > batch) threads = (int)batch;
     pthread_t *tid = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
@@ -381,6 +407,8 @@ static void run_columns(int field, int dir, size_t n, uint32_t batch, uint32_t *
         jobs[t].field = field;
         jobs[t].dir = dir;
         jobs[t].n = n;
+        jobs[t].log_n = log_n;
+        jobs[t].tw = tw;
         jobs[t].data = soa;
         jobs[t].c0 = (uint32_t)((uint64_t)batch * t / threads);
         jobs[t].c1 = (uint32_t)((uint64_t)batch * (t + 1) / threads);
@@ -393,6 +421,7 @@ static void run_columns(int field, int dir, size_t n, uint32_t batch, uint32_t *
     }
     free(tid);
     free(jobs);
+    free(tw);
 }
 
 /* The reference's batch APIs run the single-column transform on every column
@@ -419,6 +448,8 @@ typedef struct {
     const uint32_t *in; /* SoA, n per column */
     uint32_t *out;      /* SoA, n_ext per column */
     size_t n, n_ext;
+    uint32_t log_n, log_ext;
+    const uint32_t *itw, *ftw; /* inverse size-n and forward size-n_ext tables, built once */
     const uint32_t *shift_pow;
     uint32_t c0, c1;
 } lde_job;
@@ -430,10 +461,10 @@ static void *lde_worker(void *arg) {
     for (uint32_t c = j->c0; c < j->c1; ++c) {
         uint32_t *dst = j->out + (size_t)c * j->n_ext;
         memcpy(dst, j->in + (size_t)c * j->n, j->n * sizeof(uint32_t));
-        orc_bb_inverse(dst, j->n);
+        if (j->n > 1) bb_inverse_tw(dst, j->n, j->log_n, j->itw);
         memset(dst + j->n, 0, (j->n_ext - j->n) * sizeof(uint32_t));
         for (size_t i = 0; i < j->n_ext; ++i) dst[i] = orc_bb_mul(dst[i], j->shift_pow[i]);
-        orc_bb_forward(dst, j->n_ext);
+        if (j->n_ext > 1) bb_forward_tw(dst, j->n_ext, j->log_ext, j->ftw);
     }
     return NULL;
 }
@@ -453,6 +484,10 @@ int orc_coset_lde_batch(const uint32_t *in, uint32_t log_n, uint32_t batch,
         pw[i] = cur;
         cur = orc_bb_mul(cur, shift);
     }
+    uint32_t *itw = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *ftw = (uint32_t *)malloc(n_ext * sizeof(uint32_t));
+    orc_bb_twiddles(log_n, 1, itw);
+    orc_bb_twiddles(log_ext, 0, ftw);
     const uint32_t *src = in;
     uint32_t *soa_in = NULL, *soa_out = out;
     if (layout == 1) {
@@ -470,6 +505,10 @@ int orc_coset_lde_batch(const uint32_t *in, uint32_t log_n, uint32_t batch,
         jobs[t].out = soa_out;
         jobs[t].n = n;
         jobs[t].n_ext = n_ext;
+        jobs[t].log_n = log_n;
+        jobs[t].log_ext = log_ext;
+        jobs[t].itw = itw;
+        jobs[t].ftw = ftw;
         jobs[t].shift_pow = pw;
         jobs[t].c0 = (uint32_t)((uint64_t)batch * t / threads);
         jobs[t].c1 = (uint32_t)((uint64_t)batch * (t + 1) / threads);
@@ -488,6 +527,8 @@ int orc_coset_lde_batch(const uint32_t *in, uint32_t log_n, uint32_t batch,
     free(tid);
     free(jobs);
     free(pw);
+    free(itw);
+    free(ftw);
     return 0;
 }
 

@@ -1,0 +1,103 @@
+// Device field arithmetic for gfx950 (VALU only; 32-bit lanes).
+//
+// M31 (p = 2^31 - 1): canonical u32 values, Mersenne folding
+//   (src/field/m31.rs:101-115, shaders/fields/m31_field.metal:18-37).
+// BabyBear (p = 2^31 - 2^27 + 1): raw Montgomery values (R = 2^32)
+//   (src/field/babybear.rs:40-51, shaders/fields/babybear_field.metal:23-43).
+//
+// Twiddle multiplications use a precomputed companion (Shoup, "NTL: A Library
+// for doing Number Theory", MulModPrecon): for a constant w in [0, p) and
+// wq = floor(w * 2^32 / p), q = mulhi(x, wq) and r = x*w - q*p (mod 2^32)
+// lies in [0, 2p) for every 32-bit x.  For BabyBear the raw Montgomery value
+// x = X*R is multiplied by the *standard* value of the twiddle, which gives
+// the raw Montgomery form of X*W directly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfn {
+
+// a in [0, 2p) -> [0, p): when a < p the subtraction wraps above a.
+template <uint32_t P>
+__device__ __forceinline__ uint32_t canon(uint32_t a) {
+  return min(a, a - P);
+}
+
+struct M31 {
+  static constexpr uint32_t P = 0x7FFFFFFFu;
+  static constexpr int ID = 0;
+  using Tw = uint32_t;  // twiddles are plain canonical values
+
+  __device__ static __forceinline__ uint32_t add(uint32_t a, uint32_t b) { return canon<P>(a + b); }
+  __device__ static __forceinline__ uint32_t sub(uint32_t a, uint32_t b) { return canon<P>(a + P - b); }
+  // a, b < p: t < p^2, (t & p) + (t >> 31) < 2p.
+  __device__ static __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t lo = (uint32_t)t & P;
+    uint32_t hi = (uint32_t)(t >> 31);
+    return canon<P>(lo + hi);
+  }
+  __device__ static __forceinline__ uint32_t mul_tw(uint32_t a, Tw w) { return mul(a, w); }
+};
+
+struct BabyBear {
+  static constexpr uint32_t P = 0x78000001u;
+  static constexpr uint32_t PINV_NEG = 0x77FFFFFFu;  // -p^-1 mod 2^32
+  static constexpr int ID = 1;
+  using Tw = uint2;  // .x = standard value w, .y = floor(w * 2^32 / p)
+
+  __device__ static __forceinline__ uint32_t add(uint32_t a, uint32_t b) { return canon<P>(a + b); }
+  __device__ static __forceinline__ uint32_t sub(uint32_t a, uint32_t b) { return canon<P>(a + P - b); }
+  // q * p mod 2^32 with p = 2^31 - 2^27 + 1, using shifts instead of a
+  // quarter-rate 32-bit multiply.
+  __device__ static __forceinline__ uint32_t mulp(uint32_t q) { return q + (q << 31) - (q << 27); }
+  // x any 32-bit value; result canonical.
+  __device__ static __forceinline__ uint32_t mul_tw(uint32_t x, Tw w) {
+    uint32_t q = __umulhi(x, w.y);
+    uint32_t r = x * w.x - mulp(q);
+    return canon<P>(r);
+  }
+  // Lazy variant: result in [0, 2p).
+  __device__ static __forceinline__ uint32_t mul_tw_lazy(uint32_t x, Tw w) {
+    uint32_t q = __umulhi(x, w.y);
+    return x * w.x - mulp(q);
+  }
+  // Montgomery product of two raw values (src/field/babybear.rs:133-137).
+  __device__ static __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * PINV_NEG;
+    uint64_t s = t + (uint64_t)m * P;  // < 2^64 for a, b < p
+    return canon<P>((uint32_t)(s >> 32));
+  }
+};
+
+// ---- butterflies ----
+// CT (decimation in time):  (x, y) -> (x + w*y, x - w*y)
+// GS (decimation in freq.): (x, y) -> (x + y, (x - y)*w)
+template <class F>
+__device__ __forceinline__ void bfly_ct(uint32_t &x, uint32_t &y, typename F::Tw w) {
+  uint32_t t = F::mul_tw(y, w);
+  uint32_t a = x;
+  x = F::add(a, t);
+  y = F::sub(a, t);
+}
+
+template <class F>
+__device__ __forceinline__ void bfly_gs(uint32_t &x, uint32_t &y, typename F::Tw w);
+
+template <>
+__device__ __forceinline__ void bfly_gs<M31>(uint32_t &x, uint32_t &y, M31::Tw w) {
+  uint32_t a = x, b = y;
+  x = M31::add(a, b);
+  y = M31::mul(M31::sub(a, b), w);
+}
+
+template <>
+__device__ __forceinline__ void bfly_gs<BabyBear>(uint32_t &x, uint32_t &y, BabyBear::Tw w) {
+  uint32_t a = x, b = y;
+  x = BabyBear::add(a, b);
+  // a + p - b is in (0, 2p): Shoup accepts any 32-bit input.
+  y = BabyBear::mul_tw(a + BabyBear::P - b, w);
+}
+
+}  // namespace sfn

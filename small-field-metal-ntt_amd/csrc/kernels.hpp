@@ -1,0 +1,81 @@
+// Kernel argument blocks and launchers shared by kernels.hip and the runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfn {
+
+constexpr int kPassThreads = 256;  // workgroup size of the pass kernels
+constexpr int kMaxLogTile = 11;    // largest LDS sub-transform (2^11 points)
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
+// Lane enumeration orders of a pass (a lane = one sub-transform instance).
+enum LaneOrder : uint32_t {
+  kLowFastest = 0,  // lanes walk the low row bits, then columns, then high bits
+  kColFastest = 1,  // lanes walk columns, then low bits, then high bits
+  kHighFastest = 2  // lanes walk the high row bits, then columns (needs lo == 0)
+};
+
+// One pass = layers [lo, lo + LOG_T) of a transform of 2^log_n rows, on every
+// column.  Row r of column c sits at c*cs + r*rs (SoA: cs = n, rs = 1;
+// row-major: cs = 1, rs = batch).
+struct PassArgs {
+  const uint32_t *src;
+  uint32_t *dst;
+  uint64_t src_cs, src_rs;
+  uint64_t dst_cs, dst_rs;
+  uint64_t instances;   // batch << (log_n - LOG_T)
+  uint32_t log_n;
+  uint32_t lo;
+  uint32_t batch;
+  uint32_t log_w;       // lanes per workgroup = 1 << log_w
+  uint32_t order;       // LaneOrder
+  uint32_t lane_contig; // 1: consecutive lanes are adjacent in memory; 0: points are
+  const void *tw;       // BabyBear: position table (Shoup pairs, layer t at 2^t - 1);
+                        // M31: the transform's block table (layer l at n - 2^(log_n-l))
+  const uint2 *twist;   // BabyBear four-step twist [(p << lo) | low], or null
+  uint32_t has_scale;   // multiply every stored value by `scale` (1/n)
+  uint32_t scale_w, scale_q;
+  // coset-LDE middle pass only
+  const uint2 *mid;       // blocks tables of 2^log_n Shoup pairs
+  const uint2 *twist_fwd; // forward twist of the blocks' high pass
+  uint32_t blocks;
+  uint64_t dst_block_off; // element offset between consecutive output blocks
+};
+
+// V1 (one launch per layer) arguments.
+struct LayerArgs {
+  const uint32_t *src;
+  uint32_t *dst;
+  uint64_t src_cs, src_rs, dst_cs, dst_rs;
+  uint64_t butterflies;  // batch << (log_n - 1)
+  uint32_t log_n;
+  uint32_t layer;
+  uint32_t batch;
+  const void *tw;  // BabyBear: full position table; M31: block table
+  uint32_t has_scale, scale_w, scale_q;
+};
+
+constexpr size_t kLaneInfoBytes = 32;  // per-lane decode record kept in LDS
+
+// Dynamic LDS of a pass: [lane records][BabyBear small table][tile, W x (T + T/32)]
+inline size_t pass_lds_bytes(int field, int log_t, uint32_t log_w) {
+  const size_t T = (size_t)1 << log_t;
+  const size_t TP = T + (T >> 5);
+  return (kLaneInfoBytes << log_w) + (field == 1 ? T * 8 : 0) + (TP << log_w) * 4;
+}
+// LDE middle pass: [lane records][inverse + forward small tables][tile]
+inline size_t lde_mid_lds_bytes(int log_t, uint32_t log_w) {
+  const size_t T = (size_t)1 << log_t;
+  const size_t TP = T + (T >> 5);
+  return (kLaneInfoBytes << log_w) + 2 * T * 8 + (TP << log_w) * 4;
+}
+hipError_t launch_pass(int field, int dir, int log_t, int logr, const PassArgs &a,
+                       uint64_t tiles, hipStream_t s);
+hipError_t launch_lde_mid(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s);
+hipError_t launch_layer(int field, int dir, const LayerArgs &a, hipStream_t s);
+hipError_t launch_pointwise(int field, const uint32_t *a, const uint32_t *b, uint32_t *out,
+                            uint64_t len, hipStream_t s);
+hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);
+
+}  // namespace sfn

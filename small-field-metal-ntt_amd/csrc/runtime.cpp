@@ -1,0 +1,833 @@
+// Host runtime behind the C ABI: context, device-resident tables, planner,
+// executor, and the host-pointer entry points.
+#include "runtime.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+
+#include "tables.hpp"
+
+namespace sfn {
+
+// ------------------------------------------------------------------ errors
+sfn_status Context::fail(sfn_status s, const std::string &msg) {
+  last_error = msg;
+  return s;
+}
+
+sfn_status Context::hip_fail(hipError_t e, const char *what) {
+  last_error = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SFN_ERR_BUFFER_ALLOC_FAILED;
+  return SFN_ERR_GPU_EXECUTION;
+}
+
+#define SFN_HIP(call, what)                              \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail(e_, what);     \
+  } while (0)
+
+Context::~Context() {
+  auto rel = [](DevBuf &b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  };
+  rel(bb_pos[0]);
+  rel(bb_pos[1]);
+  rel(bb_small_pair);
+  for (auto &kv : bb_twist) rel(kv.second);
+  for (auto &kv : bb_lde_mid) rel(kv.second);
+  for (auto &kv : m31_tw) rel(kv.second);
+  rel(scratch_in);
+  rel(scratch_out);
+  rel(scratch_pp);
+  for (auto e : tev) (void)hipEventDestroy(e);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+sfn_status Context::ensure(DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return SFN_OK;
+  if (b.ptr) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return SFN_OK;
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(SFN_ERR_BUFFER_ALLOC_FAILED, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.bytes = bytes;
+  return SFN_OK;
+}
+
+sfn_status Context::upload_table(DevBuf &b, const void *host, size_t bytes) {
+  sfn_status s = ensure(b, bytes);
+  if (s) return s;
+  SFN_HIP(hipMemcpy(b.ptr, host, bytes, hipMemcpyHostToDevice), "table upload");
+  table_bytes += bytes;
+  return SFN_OK;
+}
+
+hipError_t Context::timing_mark(hipStream_t s, bool before, const std::string &label) {
+  if (!pass_timing || s != stream) return hipSuccess;
+  const size_t need = 2 * (size_t)(tcount + 1);
+  while (tev.size() < need) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    tev.push_back(e);
+  }
+  if (before) {
+    if (tlabel.size() <= tcount) tlabel.resize(tcount + 1);
+    tlabel[tcount] = label;
+    return hipEventRecord(tev[2 * tcount], s);
+  }
+  hipError_t r = hipEventRecord(tev[2 * tcount + 1], s);
+  ++tcount;
+  return r;
+}
+
+// ------------------------------------------------------------------ tables
+sfn_status Context::bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  if (log_max < kMaxLogTile) log_max = kMaxLogTile;
+  if (bb_pos_log < log_max) {
+    for (int d = 0; d < 2; ++d) {
+      auto t = host::bb_position_table(log_max, d == 1);
+      if (bb_pos[d].bytes) table_bytes -= bb_pos[d].bytes;
+      sfn_status s = upload_table(bb_pos[d], t.data(), t.size() * sizeof(host::Pair));
+      if (s) return s;
+    }
+    bb_pos_log = log_max;
+  }
+  if (fwd) *fwd = (const uint2 *)bb_pos[0].ptr;
+  if (inv) *inv = (const uint2 *)bb_pos[1].ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_small_pair_table(const uint2 **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  if (!bb_small_pair.ptr) {
+    auto f = host::bb_position_table(kMaxLogTile, false);
+    auto i = host::bb_position_table(kMaxLogTile, true);
+    std::vector<host::Pair> both(i);
+    both.insert(both.end(), f.begin(), f.end());
+    sfn_status s = upload_table(bb_small_pair, both.data(), both.size() * sizeof(host::Pair));
+    if (s) return s;
+  }
+  *out = (const uint2 *)bb_small_pair.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto key = std::make_tuple(lo, hi, (int)inverse);
+  auto it = bb_twist.find(key);
+  if (it == bb_twist.end()) {
+    auto t = host::bb_twist_table(lo, hi, inverse);
+    DevBuf b;
+    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(host::Pair));
+    if (s) return s;
+    it = bb_twist.emplace(key, b).first;
+  }
+  *out = (const uint2 *)it->second.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto key = std::make_tuple(log_n, added_bits, shift_std);
+  auto it = bb_lde_mid.find(key);
+  if (it == bb_lde_mid.end()) {
+    auto t = host::bb_lde_mid_table(log_n, added_bits, shift_std);
+    DevBuf b;
+    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(host::Pair));
+    if (s) return s;
+    it = bb_lde_mid.emplace(key, b).first;
+  }
+  *out = (const uint2 *)it->second.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::m31_table(uint32_t log_n, bool inverse, const uint32_t **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto key = std::make_pair(log_n, (int)inverse);
+  auto it = m31_tw.find(key);
+  if (it == m31_tw.end()) {
+    auto t = host::m31_block_table(log_n, inverse);
+    DevBuf b;
+    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
+    if (s) return s;
+    it = m31_tw.emplace(key, b).first;
+  }
+  *out = (const uint32_t *)it->second.ptr;
+  return SFN_OK;
+}
+
+// ------------------------------------------------------------------ validation
+// Mirrors the reference's checks: power-of-two sizes via log_n, len == n*batch
+// (implicit in the ABI), log_n <= 30 (src/ntt/metal_r2.rs:78) and BabyBear's
+// two-adicity 27 (src/field/babybear.rs:70).
+sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t log_n, uint32_t batch,
+                        int layout) {
+  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR)
+    return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
+  if (variant < SFN_CT_DIT_R2 || variant > SFN_VARIANT_AUTO)
+    return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown variant");
+  if (dir != SFN_FORWARD && dir != SFN_INVERSE) return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown direction");
+  if (layout != SFN_LAYOUT_SOA && layout != SFN_LAYOUT_ROW_MAJOR)
+    return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown layout");
+  if (batch == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "batch must be >= 1");
+  const uint32_t max_log = field == SFN_FIELD_M31 ? 30 : 27;
+  if (log_n > max_log) return ctx->fail(SFN_ERR_INVALID_SIZE, "log_n " + std::to_string(log_n) + " exceeds the field limit");
+  if (((uint64_t)batch << log_n) > (1ull << 34)) return ctx->fail(SFN_ERR_INVALID_SIZE, "transform too large");
+  return SFN_OK;
+}
+
+sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout) {
+  if (layout != SFN_LAYOUT_SOA && layout != SFN_LAYOUT_ROW_MAJOR)
+    return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown layout");
+  if (batch == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "batch must be >= 1");
+  if (log_n == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "coset LDE needs log_n >= 1");
+  // shift = two_adic_generator(log_n + added_bits + 1) needs <= 27 (src/lde/mod.rs:59)
+  if (log_n + added_bits + 1 > 27) return ctx->fail(SFN_ERR_INVALID_SIZE, "log_n + added_bits + 1 exceeds 27");
+  if (((uint64_t)batch << (log_n + added_bits)) > (1ull << 34)) return ctx->fail(SFN_ERR_INVALID_SIZE, "LDE too large");
+  return SFN_OK;
+}
+
+// ------------------------------------------------------------------ planning
+Schedule Context::schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const {
+  auto it = tuned.find(TuneKey{field, log_n, batch, layout});
+  if (it != tuned.end()) return it->second;
+  Schedule s;
+  // keep at least ~1024 workgroups per pass: small single columns get smaller tiles
+  const uint64_t elems = (uint64_t)batch << log_n;
+  if (elems < (1ull << 21)) s.max_log_t = 7;
+  else if (elems < (1ull << 24)) s.max_log_t = 8;
+  return s;
+}
+
+namespace {
+
+struct Layout {
+  uint64_t cs, rs;
+};
+
+inline Layout layout_of(int layout, uint32_t log_rows, uint32_t batch) {
+  if (layout == SFN_LAYOUT_SOA) return Layout{(uint64_t)1 << log_rows, 1};
+  return Layout{1, batch};
+}
+
+// Splits [0, log_n) into passes of at most max_log_t layers, as even as possible.
+std::vector<std::pair<uint32_t, uint32_t>> split_layers(uint32_t log_n, uint32_t max_log_t) {
+  std::vector<std::pair<uint32_t, uint32_t>> r;  // (lo, log_t), ascending lo
+  if (log_n == 0) return r;
+  const uint32_t np = (log_n + max_log_t - 1) / max_log_t;
+  uint32_t lo = 0;
+  for (uint32_t i = 0; i < np; ++i) {
+    const uint32_t lt = (log_n - lo) / (np - i) + (((log_n - lo) % (np - i)) ? 1 : 0);
+    r.emplace_back(lo, lt);
+    lo += lt;
+  }
+  return r;
+}
+
+// Lane geometry of one pass over layers [lo, lo + log_t).
+void choose_lanes(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) {
+  const uint32_t max_log_w_lds = p.log_t >= 14 ? 0 : 14 - p.log_t;  // W * T <= 16384
+  uint32_t lw = want_log_w;
+  if (layout == SFN_LAYOUT_ROW_MAJOR) {
+    p.order = kColFastest;
+    p.lane_contig = 1;
+    if (p.log_t <= 6) lw = 6;  // small tiles: more lanes
+  } else if (p.lo > 0) {
+    p.order = kLowFastest;
+    p.lane_contig = 1;
+    if (p.log_t <= 6) lw = 6;
+    if (lw > p.lo) lw = p.lo;  // lanes must be adjacent rows of one column
+  } else {
+    p.lane_contig = 0;
+    p.order = batch >= (1u << 2) ? kColFastest : kHighFastest;
+    lw = p.log_t >= 10 ? 2 : (p.log_t >= 8 ? 3 : 5);
+  }
+  if (lw > max_log_w_lds) lw = max_log_w_lds;
+  if (lw > 8) lw = 8;
+  p.log_w = lw;
+  const uint64_t instances = (uint64_t)batch << (p.log_n - p.log_t);
+  p.tiles = (instances + (1ull << lw) - 1) >> lw;
+}
+
+}  // namespace
+
+static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src,
+                              uint32_t *dst, Layout ls, Layout ld, uint32_t batch, uint32_t log_n_real,
+                              hipStream_t s) {
+  if (p.layer_kernel) {
+    LayerArgs a{};
+    a.src = src;
+    a.dst = dst;
+    a.src_cs = ls.cs;
+    a.src_rs = ls.rs;
+    a.dst_cs = ld.cs;
+    a.dst_rs = ld.rs;
+    a.log_n = p.log_n;
+    a.layer = p.layer;
+    a.batch = batch;
+    a.butterflies = (uint64_t)batch << (p.log_n - 1);
+    if (field == SFN_FIELD_BABYBEAR) {
+      const uint2 *f, *i;
+      sfn_status st = ctx->bb_position(p.log_n, &f, &i);
+      if (st) return st;
+      a.tw = dir == 0 ? f : i;
+    } else {
+      const uint32_t *t;
+      sfn_status st = ctx->m31_table(p.log_n, dir == 1, &t);
+      if (st) return st;
+      a.tw = t;
+    }
+    a.has_scale = 0;
+    if (p.scale) {
+      a.has_scale = 1;
+      const uint64_t n = 1ull << log_n_real;
+      if (field == SFN_FIELD_BABYBEAR) {
+        host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
+        a.scale_w = pr.w;
+        a.scale_q = pr.q;
+      } else {
+        a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
+      }
+    }
+    (void)ctx->timing_mark(s, true, "k_layer");
+    hipError_t e = launch_layer(field, dir, a, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "layer kernel launch");
+    (void)ctx->timing_mark(s, false, "");
+    return SFN_OK;
+  }
+  PassArgs a{};
+  a.src = src;
+  a.dst = dst;
+  a.src_cs = ls.cs;
+  a.src_rs = ls.rs;
+  a.dst_cs = ld.cs;
+  a.dst_rs = ld.rs;
+  a.log_n = p.log_n;
+  a.lo = p.lo;
+  a.batch = batch;
+  a.log_w = p.log_w;
+  a.order = p.order;
+  a.lane_contig = p.lane_contig;
+  a.instances = (uint64_t)batch << (p.log_n - p.log_t);
+  if (field == SFN_FIELD_BABYBEAR) {
+    const uint2 *f, *i;
+    sfn_status st = ctx->bb_position(kMaxLogTile, &f, &i);
+    if (st) return st;
+    a.tw = dir == 0 ? f : i;
+    if (p.lo > 0) {
+      const uint2 *tw;
+      st = ctx->bb_twist_table(p.lo, p.lo + p.log_t, dir == 1, &tw);
+      if (st) return st;
+      a.twist = tw;
+    }
+  } else {
+    const uint32_t *t;
+    sfn_status st = ctx->m31_table(p.log_n, dir == 1, &t);
+    if (st) return st;
+    a.tw = t;
+  }
+  if (p.scale) {
+    a.has_scale = 1;
+    const uint64_t n = 1ull << log_n_real;
+    if (field == SFN_FIELD_BABYBEAR) {
+      host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
+      a.scale_w = pr.w;
+      a.scale_q = pr.q;
+    } else {
+      a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
+    }
+  }
+  (void)ctx->timing_mark(s, true, std::string("k_pass<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
+                                     std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
+                                     std::to_string(p.lo));
+  hipError_t e = launch_pass(field, dir, p.log_t, p.logr, a, p.tiles, s);
+  if (e != hipSuccess) return ctx->hip_fail(e, "pass kernel launch");
+  (void)ctx->timing_mark(s, false, "");
+  return SFN_OK;
+}
+
+sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
+                            const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force) {
+  const uint64_t total = (uint64_t)batch << log_n;
+  if (log_n == 0) {  // identity (src/ntt/metal_r2.rs:81-83)
+    if (d_in != d_out) {
+      hipError_t e = launch_copy(d_in, d_out, total, s);
+      if (e != hipSuccess) return hip_fail(e, "copy launch");
+    }
+    return SFN_OK;
+  }
+  timing_begin_sequence(s);
+  Schedule sc = force ? *force : schedule_for(field, log_n, batch, layout);
+  std::vector<PassDesc> passes;
+  const bool fwd = dir == SFN_FORWARD;
+  if (variant == SFN_CT_DIT_R2) {
+    for (uint32_t k = 0; k < log_n; ++k) {
+      PassDesc p;
+      p.layer_kernel = true;
+      p.layer = fwd ? log_n - 1 - k : k;
+      p.log_n = log_n;
+      passes.push_back(p);
+    }
+  } else {
+    uint32_t logr = sc.logr;
+    if (variant == SFN_CT_GS_R2 || variant == SFN_STOCKHAM_R2) logr = 1;
+    if (variant == SFN_CT_GS_R4) logr = 2;
+    uint32_t mlt = sc.max_log_t;
+    if (mlt > kMaxLogTile) mlt = kMaxLogTile;
+    auto sp = split_layers(log_n, mlt);
+    if (fwd) std::reverse(sp.begin(), sp.end());  // forward runs the high layers first
+    for (auto &pr : sp) {
+      PassDesc p;
+      p.lo = pr.first;
+      p.log_t = (int)pr.second;
+      p.logr = (int)std::min<uint32_t>(logr, pr.second);
+      p.log_n = log_n;
+      choose_lanes(p, layout, batch, sc.log_w);
+      passes.push_back(p);
+    }
+  }
+  if (!fwd) passes.back().scale = true;  // 1/n folded into the last inverse pass
+  // buffers: pass 0 reads the input; V3 alternates between out and scratch and
+  // ends in out; the others work in place on out.
+  const size_t bytes = total * sizeof(uint32_t);
+  uint32_t *pp = nullptr;
+  if (variant == SFN_STOCKHAM_R2 && passes.size() > 1) {
+    sfn_status st = ensure(scratch_pp, bytes);
+    if (st) return st;
+    pp = (uint32_t *)scratch_pp.ptr;
+  }
+  const Layout L = layout_of(layout, log_n, batch);
+  const uint32_t *src = d_in;
+  for (size_t i = 0; i < passes.size(); ++i) {
+    uint32_t *dst = d_out;
+    if (pp) dst = ((passes.size() - 1 - i) % 2 == 0) ? d_out : pp;
+    sfn_status st = launch_desc(this, field, dir, passes[i], src, dst, L, L, batch, log_n, s);
+    if (st) return st;
+    src = dst;
+  }
+  return SFN_OK;
+}
+
+sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw,
+                            int layout, const uint32_t *d_in, uint32_t *d_out, hipStream_t s,
+                            const Schedule *force) {
+  const uint32_t LN = log_n + added_bits;
+  const uint32_t shift_std =
+      shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
+  timing_begin_sequence(s);
+  Schedule sc = force ? *force : schedule_for(2 /*lde*/, log_n, batch, layout);
+  // middle pass covers [lo, log_n); K1 and K3 cover [0, lo)
+  uint32_t mid_t = log_n <= sc.max_log_t ? log_n : (log_n + 1) / 2;
+  if (mid_t > sc.max_log_t) mid_t = sc.max_log_t;
+  if (log_n - mid_t > sc.max_log_t) return fail(SFN_ERR_UNSUPPORTED, "LDE needs log_n <= 2 * max tile");
+  const uint32_t lo = log_n - mid_t;
+  const Layout Lin = layout_of(layout, log_n, batch);
+  const Layout Lout = layout_of(layout, LN, batch);
+  sfn_status st;
+  // K1: inverse layers [0, lo) (no normalisation) into the first n rows of out
+  const uint32_t *mid_src = d_in;
+  Layout mid_ls = Lin;
+  if (lo > 0) {
+    PassDesc p;
+    p.lo = 0;
+    p.log_t = (int)lo;
+    p.logr = (int)std::min<uint32_t>(sc.logr, lo);
+    p.log_n = log_n;
+    choose_lanes(p, layout, batch, sc.log_w);
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, d_in, d_out, Lin, Lout, batch, log_n, s);
+    if (st) return st;
+    mid_src = d_out;
+    mid_ls = Lout;
+  }
+  // K2: inverse [lo, log_n) + n^-1 (shift*zeta_b)^j + forward [log_n-1, lo] per block
+  {
+    PassArgs a{};
+    a.src = mid_src;
+    a.dst = d_out;
+    a.src_cs = mid_ls.cs;
+    a.src_rs = mid_ls.rs;
+    a.dst_cs = Lout.cs;
+    a.dst_rs = Lout.rs;
+    a.log_n = log_n;
+    a.lo = lo;
+    a.batch = batch;
+    PassDesc p;
+    p.lo = lo;
+    p.log_t = (int)mid_t;
+    p.log_n = log_n;
+    choose_lanes(p, layout, batch, sc.mid_log_w);
+    // the middle kernel runs W << (mid_t - k) threads (one register item each):
+    // at most 1024; lanes over low row bits must stay within one column (W <= 2^lo)
+    const uint32_t k = std::min<uint32_t>(5, mid_t);
+    uint32_t lw = p.log_w;
+    while (lw > 0 && ((1u << lw) << (mid_t - k)) > 1024) --lw;
+    while (((1u << lw) << (mid_t - k)) < 256 && (p.order != kLowFastest || lw < lo) && lw < 6 &&
+           ((2u << lw) << (mid_t - k)) <= 1024)
+      ++lw;
+    a.log_w = lw;
+    a.order = p.order;
+    a.lane_contig = p.lane_contig;
+    a.instances = (uint64_t)batch << lo;
+    const uint2 *pair, *twi, *twf, *mid;
+    if ((st = bb_small_pair_table(&pair))) return st;
+    if ((st = bb_twist_table(lo, log_n, true, &twi))) return st;
+    if ((st = bb_twist_table(lo, log_n, false, &twf))) return st;
+    if ((st = bb_mid_table(log_n, added_bits, shift_std, &mid))) return st;
+    a.tw = pair;
+    a.twist = twi;
+    a.twist_fwd = twf;
+    a.mid = mid;
+    a.blocks = 1u << added_bits;
+    a.dst_block_off = ((uint64_t)1 << log_n) * Lout.rs;
+    const uint64_t tiles = (a.instances + (1ull << lw) - 1) >> lw;
+    (void)timing_mark(s, true, "k_lde_mid<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
+    hipError_t e = launch_lde_mid((int)mid_t, a, tiles, s);
+    if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
+    (void)timing_mark(s, false, "");
+  }
+  // K3: forward layers [0, lo) on every output block (the N-row array)
+  if (lo > 0) {
+    PassDesc p;
+    p.lo = 0;
+    p.log_t = (int)lo;
+    p.logr = (int)std::min<uint32_t>(sc.logr, lo);
+    p.log_n = LN;
+    choose_lanes(p, layout, batch, sc.log_w);
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s);
+    if (st) return st;
+  }
+  return SFN_OK;
+}
+
+}  // namespace sfn
+
+// ======================================================================= C ABI
+using sfn::Context;
+
+static uint64_t now_ns() {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+static hipStream_t pick_stream(Context &c, void *stream) { return stream ? (hipStream_t)stream : c.stream; }
+
+// Host-pointer call: upload, run between an event pair, download.
+template <class Run>
+static sfn_status host_call(Context &c, const uint32_t *in, size_t in_elems, uint32_t *out, size_t out_elems,
+                            bool in_place_ok, uint64_t *device_ns, Run run) {
+  (void)hipSetDevice(c.device);
+  sfn_status st = c.ensure(c.scratch_in, in_elems * 4);
+  if (st) return st;
+  uint32_t *d_in = (uint32_t *)c.scratch_in.ptr;
+  uint32_t *d_out = d_in;
+  if (!in_place_ok || out_elems != in_elems) {
+    if ((st = c.ensure(c.scratch_out, out_elems * 4))) return st;
+    d_out = (uint32_t *)c.scratch_out.ptr;
+  }
+  hipError_t e = hipMemcpyAsync(d_in, in, in_elems * 4, hipMemcpyHostToDevice, c.stream);
+  if (e != hipSuccess) return c.hip_fail(e, "upload");
+  if ((e = hipEventRecord(c.ev0, c.stream)) != hipSuccess) return c.hip_fail(e, "event");
+  if ((st = run(d_in, d_out))) return st;
+  if ((e = hipEventRecord(c.ev1, c.stream)) != hipSuccess) return c.hip_fail(e, "event");
+  e = hipMemcpyAsync(out, d_out, out_elems * 4, hipMemcpyDeviceToHost, c.stream);
+  if (e != hipSuccess) return c.hip_fail(e, "download");
+  if ((e = hipStreamSynchronize(c.stream)) != hipSuccess) return c.hip_fail(e, "GPU execution");
+  float ms = 0.f;
+  if ((e = hipEventElapsedTime(&ms, c.ev0, c.ev1)) != hipSuccess) return c.hip_fail(e, "event timing");
+  if (device_ns) *device_ns = (uint64_t)((double)ms * 1e6);
+  c.timing_pending = false;
+  return SFN_OK;
+}
+
+extern "C" {
+
+const char *sfn_status_string(sfn_status s) {
+  switch (s) {
+    case SFN_OK: return "ok";
+    case SFN_ERR_INVALID_SIZE: return "invalid size";
+    case SFN_ERR_DEVICE_NOT_FOUND: return "device not found";
+    case SFN_ERR_BUFFER_ALLOC_FAILED: return "buffer allocation failed";
+    case SFN_ERR_KERNEL_LOAD: return "kernel load error";
+    case SFN_ERR_GPU_EXECUTION: return "GPU execution error";
+    case SFN_ERR_CORRECTNESS: return "correctness mismatch";
+    case SFN_ERR_UNSUPPORTED: return "unsupported";
+    case SFN_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case SFN_ERR_BUSY: return "busy";
+    case SFN_ERR_NOT_READY: return "not ready";
+  }
+  return "unknown status";
+}
+
+sfn_status sfn_ctx_create(int device, sfn_ctx **out) {
+  if (!out) return SFN_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return SFN_ERR_DEVICE_NOT_FOUND;
+  }
+  if (hipSetDevice(device) != hipSuccess) return SFN_ERR_DEVICE_NOT_FOUND;
+  auto *c = new (std::nothrow) sfn_ctx();
+  if (!c) return SFN_ERR_BUFFER_ALLOC_FAILED;
+  Context &x = c->impl;
+  x.device = device;
+  if (hipGetDeviceProperties(&x.props, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&x.ev0) != hipSuccess || hipEventCreate(&x.ev1) != hipSuccess) {
+    delete c;
+    return SFN_ERR_GPU_EXECUTION;
+  }
+  if (strncmp(x.props.gcnArchName, "gfx950", 6) != 0) {
+    delete c;
+    return SFN_ERR_DEVICE_NOT_FOUND;  // the kernels are built for gfx950 only
+  }
+  *out = c;
+  return SFN_OK;
+}
+
+void sfn_ctx_destroy(sfn_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->impl.device);
+  (void)hipStreamSynchronize(ctx->impl.stream);
+  delete ctx;
+}
+
+const char *sfn_last_error(const sfn_ctx *ctx) { return ctx ? ctx->impl.last_error.c_str() : ""; }
+
+sfn_status sfn_device_info_get(sfn_ctx *ctx, sfn_device_info *info) {
+  if (!ctx || !info) return SFN_ERR_INVALID_ARGUMENT;
+  const hipDeviceProp_t &p = ctx->impl.props;
+  memset(info, 0, sizeof(*info));
+  strncpy(info->name, p.name, sizeof(info->name) - 1);
+  strncpy(info->gcn_arch, p.gcnArchName, sizeof(info->gcn_arch) - 1);
+  info->compute_units = p.multiProcessorCount;
+  info->lds_bytes_per_cu = (int)p.maxSharedMemoryPerMultiProcessor;
+  info->hbm_bytes = p.totalGlobalMem;
+  info->clock_khz = p.clockRate;
+  info->mem_clock_khz = p.memoryClockRate;
+  info->mem_bus_width = p.memoryBusWidth;
+  return SFN_OK;
+}
+
+sfn_status sfn_ntt(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sfn_direction dir, uint32_t log_n,
+                   uint32_t batch, sfn_layout layout, const uint32_t *in, uint32_t *out, uint64_t *device_ns) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_ntt(&c, field, variant, dir, log_n, batch, layout);
+  if (st) return st;
+  if (!in || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  const size_t elems = (size_t)batch << log_n;
+  if (log_n == 0) {  // identity, no GPU work (src/ntt/metal_r2.rs:81-83)
+    if (in != out) memmove(out, in, elems * 4);
+    if (device_ns) *device_ns = 0;
+    return SFN_OK;
+  }
+  return host_call(c, in, elems, out, elems, true, device_ns, [&](const uint32_t *di, uint32_t *dout) {
+    return c.run_ntt(field, variant, dir, log_n, batch, layout, di, dout, c.stream);
+  });
+}
+
+sfn_status sfn_coset_lde_batch(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                               uint32_t shift_raw, sfn_layout layout, const uint32_t *in, uint32_t *out,
+                               uint64_t *device_ns) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
+  if (st) return st;
+  if (!in || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  const size_t ni = (size_t)batch << log_n, no = (size_t)batch << (log_n + added_bits);
+  return host_call(c, in, ni, out, no, false, device_ns, [&](const uint32_t *di, uint32_t *dout) {
+    return c.run_lde(log_n, batch, added_bits, shift_raw, layout, di, dout, c.stream);
+  });
+}
+
+sfn_status sfn_pointwise_mul(sfn_ctx *ctx, sfn_field field, const uint32_t *a, const uint32_t *b, uint32_t *out,
+                             size_t len) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR) return c.fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
+  if (len == 0) return SFN_OK;
+  if (!a || !b || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  (void)hipSetDevice(c.device);
+  sfn_status st;
+  if ((st = c.ensure(c.scratch_in, len * 8))) return st;
+  if ((st = c.ensure(c.scratch_out, len * 4))) return st;
+  uint32_t *da = (uint32_t *)c.scratch_in.ptr, *db = da + len, *dout = (uint32_t *)c.scratch_out.ptr;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(da, a, len * 4, hipMemcpyHostToDevice, c.stream))) return c.hip_fail(e, "upload");
+  if ((e = hipMemcpyAsync(db, b, len * 4, hipMemcpyHostToDevice, c.stream))) return c.hip_fail(e, "upload");
+  if ((e = sfn::launch_pointwise(field, da, db, dout, len, c.stream))) return c.hip_fail(e, "launch");
+  if ((e = hipMemcpyAsync(out, dout, len * 4, hipMemcpyDeviceToHost, c.stream))) return c.hip_fail(e, "download");
+  if ((e = hipStreamSynchronize(c.stream))) return c.hip_fail(e, "GPU execution");
+  return SFN_OK;
+}
+
+// ---- device-resident tier ----
+sfn_status sfn_buf_alloc(sfn_ctx *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  *dptr = nullptr;
+  if (bytes == 0) return c.fail(SFN_ERR_BUFFER_ALLOC_FAILED, "zero-length allocation");  // src/gpu.rs:94-96
+  (void)hipSetDevice(c.device);
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    return c.fail(SFN_ERR_BUFFER_ALLOC_FAILED, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return SFN_OK;
+}
+
+sfn_status sfn_buf_free(sfn_ctx *ctx, void *dptr) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  if (!dptr) return SFN_OK;
+  (void)hipSetDevice(ctx->impl.device);
+  hipError_t e = hipFree(dptr);
+  if (e != hipSuccess) return ctx->impl.hip_fail(e, "hipFree");
+  return SFN_OK;
+}
+
+sfn_status sfn_upload(sfn_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick_stream(c, stream));
+  return e == hipSuccess ? SFN_OK : c.hip_fail(e, "upload");
+}
+
+sfn_status sfn_download(sfn_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pick_stream(c, stream));
+  return e == hipSuccess ? SFN_OK : c.hip_fail(e, "download");
+}
+
+sfn_status sfn_synchronize(sfn_ctx *ctx, void *stream) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  hipError_t e = hipStreamSynchronize(pick_stream(c, stream));
+  return e == hipSuccess ? SFN_OK : c.hip_fail(e, "GPU execution");
+}
+
+sfn_status sfn_twiddles_prepare(sfn_ctx *ctx, sfn_field field, uint32_t log_n) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  (void)hipSetDevice(c.device);
+  if (field == SFN_FIELD_BABYBEAR) {
+    if (log_n > 27) return c.fail(SFN_ERR_INVALID_SIZE, "log_n exceeds 27");
+    return c.bb_position(log_n, nullptr, nullptr);
+  }
+  if (field != SFN_FIELD_M31) return c.fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
+  if (log_n > 30) return c.fail(SFN_ERR_INVALID_SIZE, "log_n exceeds 30");
+  const uint32_t *t;
+  sfn_status st = c.m31_table(log_n, false, &t);
+  if (st) return st;
+  return c.m31_table(log_n, true, &t);
+}
+
+sfn_status sfn_twiddles_bytes(sfn_ctx *ctx, size_t *bytes) {
+  if (!ctx || !bytes) return SFN_ERR_INVALID_ARGUMENT;
+  *bytes = ctx->impl.table_bytes;
+  return SFN_OK;
+}
+
+sfn_status sfn_ntt_device(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sfn_direction dir, uint32_t log_n,
+                          uint32_t batch, sfn_layout layout, const uint32_t *d_in, uint32_t *d_out, void *stream) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_ntt(&c, field, variant, dir, log_n, batch, layout);
+  if (st) return st;
+  if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  (void)hipSetDevice(c.device);
+  hipStream_t s = pick_stream(c, stream);
+  const bool own = s == c.stream;
+  if (own) (void)hipEventRecord(c.ev0, s);
+  st = c.run_ntt(field, variant, dir, log_n, batch, layout, d_in, d_out, s);
+  if (own) {
+    (void)hipEventRecord(c.ev1, s);
+    c.timing_pending = st == SFN_OK;
+  }
+  return st;
+}
+
+sfn_status sfn_coset_lde_batch_device(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                                      uint32_t shift_raw, sfn_layout layout, const uint32_t *d_in, uint32_t *d_out,
+                                      void *stream) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
+  if (st) return st;
+  if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
+  (void)hipSetDevice(c.device);
+  hipStream_t s = pick_stream(c, stream);
+  const bool own = s == c.stream;
+  if (own) (void)hipEventRecord(c.ev0, s);
+  st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s);
+  if (own) {
+    (void)hipEventRecord(c.ev1, s);
+    c.timing_pending = st == SFN_OK;
+  }
+  return st;
+}
+
+sfn_status sfn_last_device_ns(sfn_ctx *ctx, uint64_t *ns) {
+  if (!ctx || !ns) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  hipError_t e = hipEventSynchronize(c.ev1);
+  if (e != hipSuccess) return c.hip_fail(e, "event");
+  float ms = 0.f;
+  e = hipEventElapsedTime(&ms, c.ev0, c.ev1);
+  if (e != hipSuccess) return c.hip_fail(e, "event timing");
+  *ns = (uint64_t)((double)ms * 1e6);
+  return SFN_OK;
+}
+
+sfn_status sfn_pass_timing(sfn_ctx *ctx, int enable) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  ctx->impl.pass_timing = enable != 0;
+  ctx->impl.tcount = 0;
+  return SFN_OK;
+}
+
+sfn_status sfn_pass_times(sfn_ctx *ctx, uint64_t *ns, uint32_t cap, uint32_t *count) {
+  if (!ctx || !count) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  *count = c.tcount;
+  for (uint32_t i = 0; i < c.tcount && i < cap && ns; ++i) {
+    hipError_t e = hipEventSynchronize(c.tev[2 * i + 1]);
+    if (e != hipSuccess) return c.hip_fail(e, "event");
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, c.tev[2 * i], c.tev[2 * i + 1]);
+    if (e != hipSuccess) return c.hip_fail(e, "event timing");
+    ns[i] = (uint64_t)((double)ms * 1e6);
+  }
+  return SFN_OK;
+}
+
+const char *sfn_pass_label(sfn_ctx *ctx, uint32_t index) {
+  if (!ctx || index >= ctx->impl.tcount || index >= ctx->impl.tlabel.size()) return "";
+  return ctx->impl.tlabel[index].c_str();
+}
+
+}  // extern "C"
+
+uint64_t sfn_now_ns() { return now_ns(); }

@@ -1,0 +1,116 @@
+// Host runtime: context, device-resident table cache, planner and executor.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sfntt.h"
+#include "kernels.hpp"
+
+namespace sfn {
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// One pass (or one V1 layer) of a plan.
+struct PassDesc {
+  bool layer_kernel = false;  // V1: one butterfly per thread for `layer`
+  uint32_t layer = 0;
+  bool lde_mid = false;
+  int log_t = 0, logr = 0;
+  uint32_t lo = 0;
+  uint32_t log_n = 0;         // rows of the (virtual) transform the pass indexes
+  uint32_t log_w = 0;
+  uint32_t order = 0, lane_contig = 0;
+  uint64_t tiles = 0;
+  int src_sel = 0, dst_sel = 0;  // 0 = caller input, 1 = caller output, 2 = scratch
+  bool scale = false;
+};
+
+struct Schedule {
+  uint32_t max_log_t = 10;  // largest LDS sub-transform
+  uint32_t logr = 5;        // register radix
+  uint32_t log_w = 4;       // lanes per workgroup (strided passes)
+  uint32_t mid_log_w = 3;   // lanes per workgroup of the LDE middle pass
+};
+
+struct TuneKey {
+  int field;
+  uint32_t log_n, batch;
+  int layout;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(field, log_n, batch, layout) < std::tie(o.field, o.log_n, o.batch, o.layout);
+  }
+};
+
+class Context {
+ public:
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing_pending = false;
+  std::string last_error;
+  hipDeviceProp_t props{};
+  std::recursive_mutex mu;  // serialises table-cache updates (server threads share it)
+
+  // device-resident tables
+  DevBuf bb_pos[2];     // position tables (fwd, inv), size 2^bb_pos_log pairs
+  uint32_t bb_pos_log = 0;
+  DevBuf bb_small_pair; // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
+  std::map<std::tuple<uint32_t, uint32_t, int>, DevBuf> bb_twist;         // (lo, hi, inverse)
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t>, DevBuf> bb_lde_mid;  // (L, a, shift_std)
+  std::map<std::pair<uint32_t, int>, DevBuf> m31_tw;                      // (L, inverse)
+  size_t table_bytes = 0;
+
+  // scratch for the host-pointer API and V3 ping-pong
+  DevBuf scratch_in, scratch_out, scratch_pp;
+
+  std::map<TuneKey, Schedule> tuned;
+
+  // per-kernel event timing of the last launch sequence on `stream`
+  bool pass_timing = false;
+  std::vector<hipEvent_t> tev;     // 2 per kernel, grown on demand
+  std::vector<std::string> tlabel; // kernel labels of the current sequence
+  uint32_t tcount = 0;
+  void timing_begin_sequence(hipStream_t s) {
+    if (s == stream) tcount = 0;
+  }
+  hipError_t timing_mark(hipStream_t s, bool before, const std::string &label);
+
+  ~Context();
+  sfn_status fail(sfn_status s, const std::string &msg);
+  sfn_status hip_fail(hipError_t e, const char *what);
+
+  sfn_status ensure(DevBuf &b, size_t bytes);
+  sfn_status upload_table(DevBuf &b, const void *host, size_t bytes);
+  sfn_status bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv);
+  sfn_status bb_small_pair_table(const uint2 **out);
+  sfn_status bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
+  sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
+  sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);
+
+  Schedule schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const;
+
+  // Plans and runs a transform on device buffers.
+  sfn_status run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
+                     const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr);
+  sfn_status run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw, int layout,
+                     const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr);
+};
+
+sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t log_n, uint32_t batch,
+                        int layout);
+sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout);
+
+}  // namespace sfn
+
+struct sfn_ctx {
+  sfn::Context impl;
+};

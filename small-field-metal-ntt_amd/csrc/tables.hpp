@@ -1,0 +1,71 @@
+// Host-side generation of twiddle, twist and coset-factor tables.
+// All arithmetic is exact modular arithmetic, so any evaluation order gives
+// the same values as the reference's generators.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace sfn {
+namespace host {
+
+constexpr uint32_t kM31P = 0x7FFFFFFFu;
+constexpr uint32_t kBBP = 0x78000001u;
+constexpr uint32_t kBBRoot27 = 440564289u;  // primitive 2^27-th root (src/field/babybear.rs:31-33)
+
+inline uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_t)a * b % p); }
+inline uint32_t powmod(uint32_t a, uint64_t e, uint32_t p) {
+  uint32_t r = 1 % p;
+  while (e) {
+    if (e & 1) r = mulmod(r, a, p);
+    a = mulmod(a, a, p);
+    e >>= 1;
+  }
+  return r;
+}
+inline uint32_t invmod(uint32_t a, uint32_t p) { return a ? powmod(a, p - 2, p) : 0; }
+
+// BabyBear standard <-> Montgomery raw (R = 2^32)
+inline uint32_t bb_to_monty(uint32_t v) { return (uint32_t)(((uint64_t)v << 32) % kBBP); }
+inline uint32_t bb_from_monty(uint32_t m) {
+  // m * R^-1 mod p; R^-1 = (2^32)^-1 mod p
+  static const uint32_t rinv = invmod((uint32_t)((1ull << 32) % kBBP), kBBP);
+  return mulmod(m, rinv, kBBP);
+}
+// Primitive 2^k-th root of unity, standard form (two_adic_generator, src/field/babybear.rs:69-76).
+inline uint32_t bb_root(uint32_t k) { return powmod(kBBRoot27, 1ull << (27 - k), kBBP); }
+
+// Shoup pair: (w, floor(w * 2^32 / p))
+struct Pair {
+  uint32_t w, q;
+};
+inline Pair shoup(uint32_t w, uint32_t p) { return Pair{w, (uint32_t)(((uint64_t)w << 32) / p)}; }
+
+inline uint64_t bitrev(uint64_t v, uint32_t bits) {
+  uint64_t r = 0;
+  for (uint32_t i = 0; i < bits; ++i) {
+    r = (r << 1) | (v & 1);
+    v >>= 1;
+  }
+  return r;
+}
+
+// BabyBear position table (src/field/babybear.rs:83-97; inverse:
+// src/ntt/babybear/twiddles.rs:13-19): layer s at offset 2^s - 1 holds
+// g_{s+1}^(+-j), j < 2^s, as Shoup pairs.  Independent of the transform size.
+std::vector<Pair> bb_position_table(uint32_t log_max, bool inverse);
+
+// BabyBear four-step twist of layers [lo, hi): entry (p << lo) | low holds
+// g_hi^(+-low * bitrev(p, hi - lo)).
+std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse);
+
+// Coset-LDE middle factors: block b (of 2^a) entry j holds
+// n^-1 * (shift * g_{L+a}^bitrev(b, a))^j, j < 2^L.
+std::vector<Pair> bb_lde_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std);
+
+// M31 circle twiddles (src/ntt/m31/twiddles.rs:19-68): layer l at offset
+// n - 2^(L-l); inverse = element-wise inverse.
+std::vector<uint32_t> m31_block_table(uint32_t log_n, bool inverse);
+
+}  // namespace host
+}  // namespace sfn

@@ -1,0 +1,202 @@
+"""GPU parity: every HIP path against the CPU oracle, bit-exact (integer work).
+
+Mirrors the reference's own GPU tests: every backend vs the CPU reference at
+sizes 2..2^16 including tile boundaries and odd log_n
+(src/ntt/m31/metal_ct_gs_r4.rs:508-539, src/ntt/test_utils.rs:61-168), batch
+== single column (src/ntt/babybear/batch.rs:438-480), and the LDE cases
+(log_n, batch, added_bits) = (8,4,1), (12,8,1), (8,4,2) with LCG seed 42
+(src/lde/mod.rs:194-271).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import sfntt
+from sfntt import api
+
+pytestmark = pytest.mark.gpu
+
+M31, BB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR
+VARIANTS = [sfntt.CT_DIT_R2, sfntt.CT_GS_R2, sfntt.STOCKHAM_R2, sfntt.CT_GS_R4, sfntt.VARIANT_AUTO]
+VNAMES = ["v1_ct_dit_r2", "v2_ct_gs_r2", "v3_stockham_r2", "v4_ct_gs_r4", "auto"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    # On the GPU box a missing device is a failure, not a skip.
+    return sfntt.HipContext(0)
+
+
+def ref_fwd(field, x):
+    return orc.m31_forward(x) if field == M31 else orc.bb_forward(x)
+
+
+def ref_inv(field, x):
+    return orc.m31_inverse(x) if field == M31 else orc.bb_inverse(x)
+
+
+def data(field, n, seed):
+    return orc.lcg_fill(seed, n, field)
+
+
+def assert_same(got, exp, what):
+    got, exp = np.asarray(got), np.asarray(exp)
+    assert got.shape == exp.shape, what
+    if not np.array_equal(got, exp):
+        bad = np.flatnonzero(got != exp)
+        i = int(bad[0])
+        raise AssertionError(f"{what}: {bad.size} mismatches, first at {i}: got {got[i]} expected {exp[i]}")
+
+
+@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+@pytest.mark.parametrize("variant", VARIANTS, ids=VNAMES)
+def test_single_column_sizes(ctx, field, variant):
+    for log_n in list(range(1, 15)) + [16]:
+        n = 1 << log_n
+        x = data(field, n, n * 11111 + 98765)
+        print(f"[parity] field={field} variant={variant} log_n={log_n}", flush=True)
+        y, _ = ctx.ntt(field, variant, sfntt.FORWARD, log_n, 1, x)
+        assert_same(y, ref_fwd(field, x), f"forward field={field} variant={variant} log_n={log_n}")
+        z, _ = ctx.ntt(field, variant, sfntt.INVERSE, log_n, 1, x)
+        assert_same(z, ref_inv(field, x), f"inverse field={field} variant={variant} log_n={log_n}")
+
+
+def test_config1_babybear_2e16_all_variants(ctx):
+    # BabyBear forward 2^16, natural-order input, seed 12345 + 2^16 (benches/ntt_benchmark.rs:654)
+    n = 1 << 16
+    x = orc.lcg_fill(12345 + n, n, BB)
+    exp = orc.bb_forward(x)
+    for cls in api.BB_VARIANTS:
+        b = cls(ctx=ctx)
+        got, _ = b.forward_ntt_gpu(x, 16)
+        assert_same(got, exp, b.name())
+
+
+def test_config2_m31_2e20_all_variants(ctx):
+    # M31 circle NTT 2^20, seed 12345 + 2^20 (benches/ntt_benchmark.rs:206)
+    n = 1 << 20
+    x = orc.lcg_fill(12345 + n, n, M31)
+    exp = orc.m31_forward(x)
+    for cls in api.M31_VARIANTS:
+        b = cls(ctx=ctx)
+        print(f"[parity] config2 {b.name()}", flush=True)
+        got, _ = b.forward_ntt_gpu(x, 20)
+        assert_same(got, exp, b.name())
+        back, _ = b.inverse_ntt_gpu(got, 20)
+        assert_same(back, x, b.name() + " roundtrip")
+
+
+@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+@pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
+def test_batches_match_single_columns(ctx, field, layout):
+    for log_n, batch in [(4, 3), (10, 5), (12, 16), (17, 4), (20, 2)]:
+        n = 1 << log_n
+        x = data(field, n * batch, 99 + log_n)
+        exp_f = orc.ntt_batch(field, False, log_n, batch, x)
+        exp_i = orc.ntt_batch(field, True, log_n, batch, x)
+        xin = x if layout == sfntt.LAYOUT_SOA else x.reshape(batch, n).T.copy().ravel()
+        for variant in (sfntt.VARIANT_AUTO, sfntt.CT_GS_R4, sfntt.CT_DIT_R2):
+            print(f"[parity] batch field={field} layout={layout} log_n={log_n} batch={batch} v={variant}",
+                  flush=True)
+            yf, _ = ctx.ntt(field, variant, sfntt.FORWARD, log_n, batch, xin, layout)
+            yi, _ = ctx.ntt(field, variant, sfntt.INVERSE, log_n, batch, xin, layout)
+            if layout == sfntt.LAYOUT_ROW_MAJOR:
+                yf = yf.reshape(n, batch).T.ravel()
+                yi = yi.reshape(n, batch).T.ravel()
+            assert_same(yf, exp_f, f"batch fwd {log_n}x{batch} v{variant}")
+            assert_same(yi, exp_i, f"batch inv {log_n}x{batch} v{variant}")
+
+
+@pytest.mark.parametrize("log_n,batch,added_bits", [(8, 4, 1), (12, 8, 1), (8, 4, 2), (1, 3, 1),
+                                                    (5, 2, 3), (14, 6, 2), (17, 3, 1)])
+@pytest.mark.parametrize("shift", [0, 31], ids=["ref_shift", "generator"])
+@pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
+def test_coset_lde_small(ctx, log_n, batch, added_bits, shift, layout):
+    n, N = 1 << log_n, 1 << (log_n + added_bits)
+    x = orc.lcg_fill(42, n * batch, BB)  # src/lde/mod.rs:194-202
+    shift_raw = 0 if shift == 0 else orc.bb_to_monty(shift)
+    exp = orc.coset_lde_batch(x, log_n, batch, added_bits, shift_raw)
+    xin = x if layout == sfntt.LAYOUT_SOA else x.reshape(batch, n).T.copy().ravel()
+    print(f"[parity] lde log_n={log_n} batch={batch} a={added_bits} shift={shift} layout={layout}", flush=True)
+    res = sfntt.CosetLdeBatch(ctx=ctx).execute(xin, log_n, batch, added_bits, shift_raw, layout)
+    got = res.data if layout == sfntt.LAYOUT_SOA else res.data.reshape(N, batch).T.ravel()
+    assert_same(got, exp, "coset LDE")
+
+
+def test_edge_cases(ctx):
+    # src/ntt/test_utils.rs:110-168
+    for cls in api.M31_VARIANTS + api.BB_VARIANTS:
+        b = cls(ctx=ctx)
+        z = np.zeros(64, np.uint32)
+        b.forward_ntt(z)
+        assert not z.any()
+        one = np.array([42], np.uint32)
+        b.forward_ntt(one)
+        assert one[0] == 42
+        b.inverse_ntt(one)
+        assert one[0] == 42
+        for bad in (np.ones(3, np.uint32), np.zeros(0, np.uint32)):
+            with pytest.raises(api.InvalidSize):
+                b.forward_ntt(bad)
+            with pytest.raises(api.InvalidSize):
+                b.inverse_ntt(bad)
+        with pytest.raises(api.InvalidSize):
+            b.forward_ntt_gpu(np.ones(8, np.uint32), 4)
+    with pytest.raises(api.InvalidSize):
+        ctx.ntt(BB, sfntt.VARIANT_AUTO, sfntt.FORWARD, 28, 1, np.zeros(1, np.uint32))
+    with pytest.raises(api.InvalidSize):
+        sfntt.CosetLdeBatch(ctx=ctx).execute(np.zeros(10, np.uint32), 4, 1, 1)
+    # M31 delta -> all ones (tests/ntt_cross_validation.rs:355-365)
+    d = np.zeros(1024, np.uint32)
+    d[0] = 1
+    y, _ = ctx.ntt(M31, sfntt.VARIANT_AUTO, sfntt.FORWARD, 10, 1, d)
+    assert (y == 1).all()
+
+
+def test_pointwise_mul_and_poly_product(ctx):
+    # src/ntt/test_utils.rs:155-168 and tests/ntt_cross_validation.rs:112-148
+    b = api.CtGsR2(ctx=ctx)
+    out = np.zeros(4, np.uint32)
+    b.pointwise_mul(np.array([2, 3, 4, 5], np.uint32), np.array([10, 20, 30, 40], np.uint32), out)
+    assert out.tolist() == [20, 60, 120, 200]
+    with pytest.raises(api.InvalidSize):
+        b.pointwise_mul(np.ones(4, np.uint32), np.ones(3, np.uint32), np.zeros(4, np.uint32))
+    bb = api.BbCtGsR4(ctx=ctx)
+    tm = orc.bb_to_monty
+    A = np.array([tm(1), tm(2), 0, 0], np.uint32)
+    B = np.array([tm(3), tm(4), 0, 0], np.uint32)
+    bb.forward_ntt(A)
+    bb.forward_ntt(B)
+    C = np.zeros(4, np.uint32)
+    bb.pointwise_mul(A, B, C)
+    bb.inverse_ntt(C)
+    assert [orc.bb_from_monty(int(v)) for v in C] == [3, 10, 8, 0]
+    rng = np.random.default_rng(5)
+    for field, p in ((M31, orc.M31_P), (BB, orc.BB_P)):
+        a = rng.integers(0, p, 100_003, dtype=np.uint64).astype(np.uint32)
+        c = rng.integers(0, p, 100_003, dtype=np.uint64).astype(np.uint32)
+        got = ctx.pointwise_mul(field, a, c)
+        mul = orc.m31_mul if field == M31 else orc.bb_mul
+        idx = rng.integers(0, a.size, 500)
+        assert all(int(got[i]) == mul(int(a[i]), int(c[i])) for i in idx)
+
+
+def test_determinism_and_device_tier(ctx):
+    log_n, batch = 18, 8
+    n = 1 << log_n
+    x = orc.lcg_fill(7, n * batch, BB)
+    r1, _ = ctx.ntt(BB, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, batch, x)
+    r2, _ = ctx.ntt(BB, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, batch, x)
+    assert np.array_equal(r1, r2)
+    din = sfntt.DeviceBuffer(ctx, n * batch)
+    dout = sfntt.DeviceBuffer(ctx, n * batch)
+    din.upload(x)
+    lib = ctx._lib
+    ctx.check(lib.sfn_ntt_device(ctx.ptr, BB, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, batch,
+                                 sfntt.LAYOUT_SOA, din.ptr, dout.ptr, None))
+    assert np.array_equal(dout.download(), r1)
+    # in place
+    ctx.check(lib.sfn_ntt_device(ctx.ptr, BB, sfntt.VARIANT_AUTO, sfntt.INVERSE, log_n, batch,
+                                 sfntt.LAYOUT_SOA, dout.ptr, dout.ptr, None))
+    assert np.array_equal(dout.download(), x)
+    assert ctx.twiddles_bytes() > 0
