@@ -183,31 +183,42 @@ typedef struct {
     sfn_field field;          /* SFN_JOB_LDE requires SFN_FIELD_BABYBEAR */
     sfn_direction dir;        /* SFN_JOB_NTT only */
     uint32_t log_n;
-    uint32_t batch;
+    uint32_t batch;           /* columns, SoA (data[col*n + row]) */
     uint32_t added_bits;      /* SFN_JOB_LDE only */
     uint32_t shift_raw;       /* SFN_JOB_LDE only; 0 = reference shift */
-    const uint32_t *host_in;  /* host input, batch << log_n values (SoA) */
-    uint32_t *host_out;       /* host output (NTT: batch << log_n; LDE: batch << (log_n+added_bits)) */
+    int device_ptrs;          /* 0: `in`/`out` are host memory (staged over PCIe);
+                                 1: they are device memory that stays resident in HBM */
+    const uint32_t *in;       /* batch << log_n values */
+    uint32_t *out;            /* NTT: batch << log_n; LDE: batch << (log_n+added_bits); must not overlap `in` */
 } sfn_request;
 
 typedef struct {
     size_t mem_budget_bytes;  /* device bytes the server may hold for job buffers (0 = 8 GiB) */
-    uint32_t max_inflight;    /* jobs on the device at once (0 = 16) */
-    uint32_t streams;         /* stream pool size (0 = 4) */
+    uint32_t max_inflight;    /* worker streams, i.e. launch sequences on the device at once (0 = 4) */
     uint32_t max_queue;       /* admission queue bound (0 = 4096) */
-    int batching;             /* 1 = group queued same-shape jobs into one launch sequence */
+    uint32_t max_group;       /* most queued same-shape jobs fused into one launch (0 = 64; 1 = no grouping) */
+    uint32_t max_group_cols;  /* column cap of one fused launch (0 = 512) */
 } sfn_server_config;
 
 typedef struct {
-    uint64_t submit_ns;   /* host monotonic clock at submit */
-    uint64_t start_ns;    /* when its device work was issued */
-    uint64_t end_ns;      /* when its result was in host memory */
+    uint64_t submit_ns;   /* host steady clock at submit */
+    uint64_t start_ns;    /* when a worker took the job */
+    uint64_t end_ns;      /* when its result was complete in `out` */
+    uint32_t group_size;  /* jobs fused into the launch that ran it */
     sfn_status status;
 } sfn_job_timing;
+
+/* Pinned host memory for request buffers (hipHostMalloc). */
+sfn_status sfn_host_alloc(sfn_ctx *ctx, size_t bytes, void **ptr);
+sfn_status sfn_host_free(sfn_ctx *ctx, void *ptr);
+/* Host steady clock in nanoseconds (the clock of sfn_job_timing). */
+uint64_t sfn_now(void);
 
 typedef struct sfn_server sfn_server;
 sfn_status sfn_server_create(sfn_ctx *ctx, const sfn_server_config *cfg, sfn_server **out);
 sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *ticket);
+/* poll: SFN_ERR_NOT_READY while running; once a finished job is reported by
+ * poll or wait its ticket is released. */
 sfn_status sfn_server_poll(sfn_server *srv, uint64_t ticket, sfn_job_timing *timing);
 sfn_status sfn_server_wait(sfn_server *srv, uint64_t ticket, sfn_job_timing *timing);
 sfn_status sfn_server_drain(sfn_server *srv);

@@ -1,25 +1,266 @@
-// Entry points whose implementation has not landed yet: they validate the
-// handle and report SFN_ERR_UNSUPPORTED.
+// Batched plans (many transforms of uneven sizes and both fields in one
+// launch per pass, replayed from a hipGraph) and the schedule autotuner.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <sstream>
+
 #include "runtime.hpp"
 
+using sfn::Context;
+using sfn::PassArgs;
+using sfn::PassDesc;
+using sfn::Schedule;
+using sfn::SegDesc;
+
+struct sfn_plan {
+  Context *ctx = nullptr;
+  struct Launch {
+    std::vector<SegDesc> segs;
+    void *d_segs = nullptr;
+    uint64_t tiles = 0;
+    size_t lds = 0;
+    uint32_t grid = 0;
+  };
+  std::vector<Launch> launches;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  uint64_t total_tiles = 0;
+
+  ~sfn_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    for (auto &l : launches)
+      if (l.d_segs) (void)hipFree(l.d_segs);
+  }
+
+  hipError_t issue(hipStream_t s) {
+    for (auto &l : launches) {
+      hipError_t e = sfn::launch_multi((const SegDesc *)l.d_segs, (uint32_t)l.segs.size(), l.tiles, l.lds, l.grid, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
 extern "C" {
-sfn_status sfn_plan_create(sfn_ctx *, const sfn_job_desc *, uint32_t, int, sfn_plan **out) {
-  if (out) *out = nullptr;
-  return SFN_ERR_UNSUPPORTED;
+
+sfn_status sfn_plan_create(sfn_ctx *cx, const sfn_job_desc *jobs, uint32_t count, int use_graph, sfn_plan **out) {
+  if (!cx || !out || (!jobs && count)) return SFN_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  Context &c = cx->impl;
+  (void)hipSetDevice(c.device);
+  auto plan = std::make_unique<sfn_plan>();
+  plan->ctx = &c;
+  for (uint32_t j = 0; j < count; ++j) {
+    const sfn_job_desc &jd = jobs[j];
+    sfn_status st = sfn::validate_ntt(&c, jd.field, SFN_VARIANT_AUTO, jd.dir, jd.log_n, jd.batch, SFN_LAYOUT_SOA);
+    if (st) return st;
+    if (!jd.d_in || !jd.d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null job buffer");
+    if (jd.log_n == 0) {
+      if (jd.d_in != jd.d_out) return c.fail(SFN_ERR_UNSUPPORTED, "out-of-place log_n == 0 job in a plan");
+      continue;  // identity in place
+    }
+    Schedule sc = c.schedule_for(jd.field, jd.log_n, jd.batch, SFN_LAYOUT_SOA);
+    auto passes = sfn::plan_ntt_passes(jd.field, SFN_VARIANT_AUTO, jd.dir, jd.log_n, jd.batch, SFN_LAYOUT_SOA, sc);
+    const uint64_t n = 1ull << jd.log_n;
+    const uint32_t *src = jd.d_in;
+    for (size_t k = 0; k < passes.size(); ++k) {
+      if (plan->launches.size() <= k) plan->launches.resize(k + 1);
+      SegDesc d{};
+      st = sfn::make_pass_args(&c, jd.field, jd.dir, passes[k], src, jd.d_out, n, 1, n, 1, jd.batch, jd.log_n, &d.a);
+      if (st) return st;
+      d.kind = (uint32_t)(jd.field * 2 + jd.dir);
+      d.log_t = (uint32_t)passes[k].log_t;
+      auto &L = plan->launches[k];
+      d.tile_begin = L.tiles;
+      L.tiles += passes[k].tiles;
+      L.lds = std::max(L.lds, sfn::pass_lds_bytes(jd.field, passes[k].log_t, passes[k].log_w));
+      L.segs.push_back(d);
+      src = jd.d_out;
+    }
+  }
+  const uint32_t cus = (uint32_t)std::max(1, c.props.multiProcessorCount);
+  for (auto &L : plan->launches) {
+    // persistent grid: enough workgroups to fill every CU a few times over
+    const uint32_t per_cu = L.lds > 80 * 1024 ? 1 : (L.lds > 40 * 1024 ? 2 : 4);
+    L.grid = (uint32_t)std::min<uint64_t>(L.tiles, (uint64_t)cus * per_cu * 2);
+    const size_t bytes = L.segs.size() * sizeof(SegDesc);
+    if (bytes) {
+      hipError_t e = hipMalloc(&L.d_segs, bytes);
+      if (e != hipSuccess) return c.hip_fail(e, "plan descriptors");
+      e = hipMemcpy(L.d_segs, L.segs.data(), bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return c.hip_fail(e, "plan descriptors");
+    }
+    plan->total_tiles += L.tiles;
+  }
+  if (use_graph) {
+    hipError_t e = hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return c.hip_fail(e, "stream");
+    e = hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return c.hip_fail(e, "graph capture");
+    hipError_t le = plan->issue(plan->capture_stream);
+    e = hipStreamEndCapture(plan->capture_stream, &plan->graph);
+    if (le != hipSuccess) return c.hip_fail(le, "graph capture launch");
+    if (e != hipSuccess) return c.hip_fail(e, "graph capture");
+    e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return c.hip_fail(e, "graph instantiate");
+    plan->use_graph = true;
+  }
+  *out = plan.release();
+  return SFN_OK;
 }
-sfn_status sfn_plan_launch(sfn_plan *, void *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_plan_info(sfn_plan *, uint32_t *, uint32_t *) { return SFN_ERR_UNSUPPORTED; }
-void sfn_plan_destroy(sfn_plan *) {}
-sfn_status sfn_autotune(sfn_ctx *, sfn_field, uint32_t, uint32_t, sfn_layout, uint32_t *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_tuning_save(sfn_ctx *, char *, size_t, size_t *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_tuning_load(sfn_ctx *, const char *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_server_create(sfn_ctx *, const sfn_server_config *, sfn_server **out) {
-  if (out) *out = nullptr;
-  return SFN_ERR_UNSUPPORTED;
+
+sfn_status sfn_plan_launch(sfn_plan *plan, void *stream) {
+  if (!plan) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = *plan->ctx;
+  hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+  hipError_t e = plan->use_graph ? hipGraphLaunch(plan->exec, s) : plan->issue(s);
+  if (e != hipSuccess) return c.hip_fail(e, "plan launch");
+  return SFN_OK;
 }
-sfn_status sfn_server_submit(sfn_server *, const sfn_request *, uint64_t *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_server_poll(sfn_server *, uint64_t, sfn_job_timing *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_server_wait(sfn_server *, uint64_t, sfn_job_timing *) { return SFN_ERR_UNSUPPORTED; }
-sfn_status sfn_server_drain(sfn_server *) { return SFN_ERR_UNSUPPORTED; }
-void sfn_server_destroy(sfn_server *) {}
+
+sfn_status sfn_plan_info(sfn_plan *plan, uint32_t *launches, uint32_t *tiles) {
+  if (!plan) return SFN_ERR_INVALID_ARGUMENT;
+  if (launches) *launches = (uint32_t)plan->launches.size();
+  if (tiles) *tiles = (uint32_t)std::min<uint64_t>(plan->total_tiles, 0xFFFFFFFFu);
+  return SFN_OK;
 }
+
+void sfn_plan_destroy(sfn_plan *plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  (void)hipDeviceSynchronize();
+  delete plan;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Candidates: largest LDS sub-transform (pass split), register radix and lanes
+// per workgroup (and the LDE middle pass's lanes).  Each is timed with a
+// hipEvent pair over a few launches on scratch buffers; the fastest is kept.
+sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t batch, sfn_layout layout,
+                        uint32_t *best_log_tile) {
+  if (!cx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = cx->impl;
+  const bool lde = (int)field == 2;  // field 2 = the coset LDE at 2x
+  sfn_status st = lde ? sfn::validate_lde(&c, log_n, batch, 1, layout)
+                      : sfn::validate_ntt(&c, field, SFN_VARIANT_AUTO, SFN_FORWARD, log_n, batch, layout);
+  if (st) return st;
+  if (log_n < 2) {
+    if (best_log_tile) *best_log_tile = log_n;
+    return SFN_OK;
+  }
+  (void)hipSetDevice(c.device);
+  const uint64_t ni = (uint64_t)batch << log_n, no = lde ? ni * 2 : ni;
+  if ((st = c.ensure(c.scratch_in, ni * 4))) return st;
+  if ((st = c.ensure(c.scratch_out, no * 4))) return st;
+  (void)hipMemsetAsync(c.scratch_in.ptr, 0, ni * 4, c.stream);
+  std::vector<Schedule> cands;
+  for (uint32_t mlt = 6; mlt <= 11; ++mlt) {
+    if ((log_n + mlt - 1) / mlt > 3) continue;
+    if (lde && (log_n > 2 * mlt)) continue;
+    for (uint32_t logr : {3u, 4u, 5u})
+      for (uint32_t lw : {2u, 3u, 4u, 5u}) {
+        Schedule s;
+        s.max_log_t = mlt;
+        s.logr = logr;
+        s.log_w = lw;
+        s.mid_log_w = lw;
+        if (mlt + lw > 15) continue;
+        cands.push_back(s);
+      }
+  }
+  float best = 1e30f;
+  Schedule bs = c.schedule_for(lde ? 2 : field, log_n, batch, layout);
+  const uint32_t *din = (const uint32_t *)c.scratch_in.ptr;
+  uint32_t *dout = (uint32_t *)c.scratch_out.ptr;
+  for (const Schedule &s : cands) {
+    auto run = [&]() {
+      return lde ? c.run_lde(log_n, batch, 1, 0, layout, din, dout, c.stream, &s)
+                 : c.run_ntt(field, SFN_VARIANT_AUTO, SFN_FORWARD, log_n, batch, layout, din, dout, c.stream, &s);
+    };
+    if (run() != SFN_OK) {  // candidate not launchable (LDS / lane limits): skip it
+      (void)hipGetLastError();
+      continue;
+    }
+    (void)hipEventRecord(c.ev0, c.stream);
+    const int reps = ni >= (1u << 24) ? 3 : 10;
+    bool ok = true;
+    for (int r = 0; r < reps; ++r) ok = ok && run() == SFN_OK;
+    (void)hipEventRecord(c.ev1, c.stream);
+    if (hipEventSynchronize(c.ev1) != hipSuccess || !ok) continue;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
+    if (ms < best) {
+      best = ms;
+      bs = s;
+    }
+  }
+  c.tuned[sfn::TuneKey{lde ? 2 : (int)field, log_n, batch, (int)layout}] = bs;
+  if (best_log_tile) *best_log_tile = bs.max_log_t;
+  c.last_error.clear();
+  return SFN_OK;
+}
+
+sfn_status sfn_tuning_save(sfn_ctx *cx, char *buf, size_t cap, size_t *len) {
+  if (!cx) return SFN_ERR_INVALID_ARGUMENT;
+  std::ostringstream o;
+  o << "[";
+  bool first = true;
+  for (auto &kv : cx->impl.tuned) {
+    if (!first) o << ",";
+    first = false;
+    o << "{\"field\":" << kv.first.field << ",\"log_n\":" << kv.first.log_n << ",\"batch\":" << kv.first.batch
+      << ",\"layout\":" << kv.first.layout << ",\"max_log_t\":" << kv.second.max_log_t
+      << ",\"logr\":" << kv.second.logr << ",\"log_w\":" << kv.second.log_w
+      << ",\"mid_log_w\":" << kv.second.mid_log_w << "}";
+  }
+  o << "]";
+  const std::string s = o.str();
+  if (len) *len = s.size() + 1;
+  if (!buf || cap < s.size() + 1) return buf ? SFN_ERR_INVALID_SIZE : SFN_OK;
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return SFN_OK;
+}
+
+// Accepts the array written by sfn_tuning_save (flat objects of integers).
+sfn_status sfn_tuning_load(sfn_ctx *cx, const char *json) {
+  if (!cx || !json) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = cx->impl;
+  const char *p = json;
+  std::map<sfn::TuneKey, Schedule> add;
+  while ((p = strchr(p, '{')) != nullptr) {
+    const char *e = strchr(p, '}');
+    if (!e) return c.fail(SFN_ERR_INVALID_ARGUMENT, "unterminated object in tuning table");
+    std::string obj(p + 1, e);
+    auto get = [&](const char *key, long *v) {
+      std::string k = std::string("\"") + key + "\":";
+      size_t i = obj.find(k);
+      if (i == std::string::npos) return false;
+      *v = strtol(obj.c_str() + i + k.size(), nullptr, 10);
+      return true;
+    };
+    long f, ln, b, lay, mlt, lr, lw, mw;
+    if (!get("field", &f) || !get("log_n", &ln) || !get("batch", &b) || !get("layout", &lay) ||
+        !get("max_log_t", &mlt) || !get("logr", &lr) || !get("log_w", &lw) || !get("mid_log_w", &mw))
+      return c.fail(SFN_ERR_INVALID_ARGUMENT, "tuning entry misses a key");
+    if (mlt < 1 || mlt > sfn::kMaxLogTile || lr < 1 || lr > 5 || lw < 0 || lw > 8 || mw < 0 || mw > 8)
+      return c.fail(SFN_ERR_INVALID_ARGUMENT, "tuning entry out of range");
+    Schedule s;
+    s.max_log_t = (uint32_t)mlt;
+    s.logr = (uint32_t)lr;
+    s.log_w = (uint32_t)lw;
+    s.mid_log_w = (uint32_t)mw;
+    add[sfn::TuneKey{(int)f, (uint32_t)ln, (uint32_t)b, (int)lay}] = s;
+    p = e + 1;
+  }
+  for (auto &kv : add) c.tuned[kv.first] = kv.second;
+  return SFN_OK;
+}
+
+}  // extern "C"

@@ -235,9 +235,9 @@ __device__ __forceinline__ uint32_t scale_mul<BabyBear>(uint32_t v, uint32_t w, 
 static_assert(sizeof(LaneInfo) == kLaneInfoBytes, "LaneInfo size");
 __host__ __device__ constexpr size_t lane_bytes(uint32_t log_w) { return kLaneInfoBytes << log_w; }
 
+// One tile (2^log_w lanes) of a pass; shared by k_pass and the multi-job kernel.
 template <class F, int DIR, int LOG_T, int LOGR>
-__global__ __launch_bounds__(NT) void k_pass(const PassArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, unsigned char *smem) {
   constexpr int T = 1 << LOG_T;
   constexpr int TP = Tile<LOG_T>::TP;
   const uint32_t log_w = a.log_w;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(NT) void k_pass(const PassArgs a) {
   uint2 *stw = reinterpret_cast<uint2 *>(smem + lane_bytes(log_w));
   uint32_t *tile = reinterpret_cast<uint32_t *>(smem + lane_bytes(log_w) + (F::ID == 1 ? T * sizeof(uint2) : 0));
 
-  const uint64_t g0 = (uint64_t)blockIdx.x << log_w;
+  const uint64_t g0 = tile_idx << log_w;
   for (uint32_t l = threadIdx.x; l < W; l += NT) decode_lane(a, LOG_T, g0 + l, lanes[l]);
   if constexpr (F::ID == 1) {
     const uint2 *gtw = reinterpret_cast<const uint2 *>(a.tw);
@@ -301,6 +301,12 @@ __global__ __launch_bounds__(NT) void k_pass(const PassArgs a) {
     if (a.has_scale) v = scale_mul<F>(v, a.scale_w, a.scale_q);
     a.dst[li.dbase + p * dps] = v;
   }
+}
+
+template <class F, int DIR, int LOG_T, int LOGR>
+__global__ __launch_bounds__(NT) void k_pass(const PassArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  pass_tile<F, DIR, LOG_T, LOGR>(a, blockIdx.x, smem);
 }
 
 #if SFN_IN_PART(4)
@@ -590,5 +596,56 @@ hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStre
 }
 
 #endif  // SFN_IN_PART(4)
+
+#if SFN_IN_PART(5)
+// ---- many jobs in one launch ----
+// Segments (one per job pass) own consecutive ranges of a global tile index;
+// a persistent grid walks the tiles with a grid stride, so every wave reaches
+// the end of the list and the launch drains.  Field, direction and tile size
+// are selected per segment (uniform across the workgroup).
+template <class F, int DIR>
+__device__ __forceinline__ void multi_dispatch(uint32_t log_t, const PassArgs &a, uint64_t t, unsigned char *smem) {
+  switch (log_t) {
+#define SFN_CASE(L) \
+  case L: pass_tile<F, DIR, L, (L < 5 ? L : 5)>(a, t, smem); break;
+    SFN_CASE(1) SFN_CASE(2) SFN_CASE(3) SFN_CASE(4) SFN_CASE(5) SFN_CASE(6)
+    SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10) SFN_CASE(11)
+#undef SFN_CASE
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_multi(const SegDesc *segs, uint32_t nseg, uint64_t total_tiles) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t s = 0;
+  for (uint64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    while (s + 1 < nseg && segs[s + 1].tile_begin <= t) ++s;
+    const PassArgs a = segs[s].a;
+    const uint64_t local = t - segs[s].tile_begin;
+    const uint32_t kind = segs[s].kind, log_t = segs[s].log_t;
+    switch (kind) {
+      case 0: multi_dispatch<M31, 0>(log_t, a, local, smem); break;
+      case 1: multi_dispatch<M31, 1>(log_t, a, local, smem); break;
+      case 2: multi_dispatch<BabyBear, 0>(log_t, a, local, smem); break;
+      case 3: multi_dispatch<BabyBear, 1>(log_t, a, local, smem); break;
+      default: break;
+    }
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+hipError_t launch_multi(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
+                        hipStream_t s) {
+  if (total_tiles == 0) return hipSuccess;
+  if (lds > kMaxLdsBytes || grid == 0) return hipErrorInvalidConfiguration;
+  static bool configured = false;
+  if (!configured) {
+    (void)hipFuncSetAttribute((const void *)k_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+    configured = true;
+  }
+  hipLaunchKernelGGL(k_multi, dim3(grid), dim3(NT), lds, s, d_segs, nseg, total_tiles);
+  return hipGetLastError();
+}
+#endif  // SFN_IN_PART(5)
 
 }  // namespace sfn

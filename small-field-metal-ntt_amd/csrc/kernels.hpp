@@ -56,6 +56,14 @@ struct LayerArgs {
   uint32_t has_scale, scale_w, scale_q;
 };
 
+// One segment of a multi-job launch: a pass of one job over its own tiles.
+struct SegDesc {
+  PassArgs a;
+  uint64_t tile_begin;  // first global tile of this segment (segments sorted)
+  uint32_t kind;        // field * 2 + direction
+  uint32_t log_t;       // sub-transform size (register radix min(5, log_t))
+};
+
 constexpr size_t kLaneInfoBytes = 32;  // per-lane decode record kept in LDS
 
 // Dynamic LDS of a pass: [lane records][BabyBear small table][tile, W x (T + T/32)]
@@ -76,6 +84,8 @@ hipError_t launch_lde_mid(int log_t, const PassArgs &a, uint64_t tiles, hipStrea
 hipError_t launch_layer(int field, int dir, const LayerArgs &a, hipStream_t s);
 hipError_t launch_pointwise(int field, const uint32_t *a, const uint32_t *b, uint32_t *out,
                             uint64_t len, hipStream_t s);
+hipError_t launch_multi(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
+                        hipStream_t s);
 hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);
 
 }  // namespace sfn

@@ -269,6 +269,90 @@ void choose_lanes(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) 
 
 }  // namespace
 
+std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
+                                      const Schedule &sc) {
+  (void)field;
+  std::vector<PassDesc> passes;
+  const bool fwd = dir == SFN_FORWARD;
+  if (variant == SFN_CT_DIT_R2) {
+    for (uint32_t k = 0; k < log_n; ++k) {
+      PassDesc p;
+      p.layer_kernel = true;
+      p.layer = fwd ? log_n - 1 - k : k;
+      p.log_n = log_n;
+      passes.push_back(p);
+    }
+  } else {
+    uint32_t logr = sc.logr;
+    if (variant == SFN_CT_GS_R2 || variant == SFN_STOCKHAM_R2) logr = 1;
+    if (variant == SFN_CT_GS_R4) logr = 2;
+    uint32_t mlt = sc.max_log_t;
+    if (mlt > kMaxLogTile) mlt = kMaxLogTile;
+    auto sp = split_layers(log_n, mlt);
+    if (fwd) std::reverse(sp.begin(), sp.end());  // forward runs the high layers first
+    for (auto &pr : sp) {
+      PassDesc p;
+      p.lo = pr.first;
+      p.log_t = (int)pr.second;
+      p.logr = (int)std::min<uint32_t>(logr, pr.second);
+      p.log_n = log_n;
+      choose_lanes(p, layout, batch, sc.log_w);
+      passes.push_back(p);
+    }
+  }
+  if (!fwd && !passes.empty()) passes.back().scale = true;  // 1/n folded into the last inverse pass
+  return passes;
+}
+
+sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src, uint32_t *dst,
+                          uint64_t src_cs, uint64_t src_rs, uint64_t dst_cs, uint64_t dst_rs, uint32_t batch,
+                          uint32_t log_n_real, PassArgs *out) {
+  PassArgs &a = *out;
+  a = PassArgs{};
+  a.src = src;
+  a.dst = dst;
+  a.src_cs = src_cs;
+  a.src_rs = src_rs;
+  a.dst_cs = dst_cs;
+  a.dst_rs = dst_rs;
+  a.log_n = p.log_n;
+  a.lo = p.lo;
+  a.batch = batch;
+  a.log_w = p.log_w;
+  a.order = p.order;
+  a.lane_contig = p.lane_contig;
+  a.instances = (uint64_t)batch << (p.log_n - p.log_t);
+  if (field == SFN_FIELD_BABYBEAR) {
+    const uint2 *f, *i;
+    sfn_status st = ctx->bb_position(kMaxLogTile, &f, &i);
+    if (st) return st;
+    a.tw = dir == 0 ? f : i;
+    if (p.lo > 0) {
+      const uint2 *tw;
+      st = ctx->bb_twist_table(p.lo, p.lo + p.log_t, dir == 1, &tw);
+      if (st) return st;
+      a.twist = tw;
+    }
+  } else {
+    const uint32_t *t;
+    sfn_status st = ctx->m31_table(p.log_n, dir == 1, &t);
+    if (st) return st;
+    a.tw = t;
+  }
+  if (p.scale) {
+    a.has_scale = 1;
+    const uint64_t n = 1ull << log_n_real;
+    if (field == SFN_FIELD_BABYBEAR) {
+      host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
+      a.scale_w = pr.w;
+      a.scale_q = pr.q;
+    } else {
+      a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
+    }
+  }
+  return SFN_OK;
+}
+
 static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src,
                               uint32_t *dst, Layout ls, Layout ld, uint32_t batch, uint32_t log_n_real,
                               hipStream_t s) {
@@ -314,47 +398,8 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     return SFN_OK;
   }
   PassArgs a{};
-  a.src = src;
-  a.dst = dst;
-  a.src_cs = ls.cs;
-  a.src_rs = ls.rs;
-  a.dst_cs = ld.cs;
-  a.dst_rs = ld.rs;
-  a.log_n = p.log_n;
-  a.lo = p.lo;
-  a.batch = batch;
-  a.log_w = p.log_w;
-  a.order = p.order;
-  a.lane_contig = p.lane_contig;
-  a.instances = (uint64_t)batch << (p.log_n - p.log_t);
-  if (field == SFN_FIELD_BABYBEAR) {
-    const uint2 *f, *i;
-    sfn_status st = ctx->bb_position(kMaxLogTile, &f, &i);
-    if (st) return st;
-    a.tw = dir == 0 ? f : i;
-    if (p.lo > 0) {
-      const uint2 *tw;
-      st = ctx->bb_twist_table(p.lo, p.lo + p.log_t, dir == 1, &tw);
-      if (st) return st;
-      a.twist = tw;
-    }
-  } else {
-    const uint32_t *t;
-    sfn_status st = ctx->m31_table(p.log_n, dir == 1, &t);
-    if (st) return st;
-    a.tw = t;
-  }
-  if (p.scale) {
-    a.has_scale = 1;
-    const uint64_t n = 1ull << log_n_real;
-    if (field == SFN_FIELD_BABYBEAR) {
-      host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
-      a.scale_w = pr.w;
-      a.scale_q = pr.q;
-    } else {
-      a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
-    }
-  }
+  sfn_status st0 = make_pass_args(ctx, field, dir, p, src, dst, ls.cs, ls.rs, ld.cs, ld.rs, batch, log_n_real, &a);
+  if (st0) return st0;
   (void)ctx->timing_mark(s, true, std::string("k_pass<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
                                      std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
                                      std::to_string(p.lo));
@@ -376,35 +421,9 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
   }
   timing_begin_sequence(s);
   Schedule sc = force ? *force : schedule_for(field, log_n, batch, layout);
-  std::vector<PassDesc> passes;
   const bool fwd = dir == SFN_FORWARD;
-  if (variant == SFN_CT_DIT_R2) {
-    for (uint32_t k = 0; k < log_n; ++k) {
-      PassDesc p;
-      p.layer_kernel = true;
-      p.layer = fwd ? log_n - 1 - k : k;
-      p.log_n = log_n;
-      passes.push_back(p);
-    }
-  } else {
-    uint32_t logr = sc.logr;
-    if (variant == SFN_CT_GS_R2 || variant == SFN_STOCKHAM_R2) logr = 1;
-    if (variant == SFN_CT_GS_R4) logr = 2;
-    uint32_t mlt = sc.max_log_t;
-    if (mlt > kMaxLogTile) mlt = kMaxLogTile;
-    auto sp = split_layers(log_n, mlt);
-    if (fwd) std::reverse(sp.begin(), sp.end());  // forward runs the high layers first
-    for (auto &pr : sp) {
-      PassDesc p;
-      p.lo = pr.first;
-      p.log_t = (int)pr.second;
-      p.logr = (int)std::min<uint32_t>(logr, pr.second);
-      p.log_n = log_n;
-      choose_lanes(p, layout, batch, sc.log_w);
-      passes.push_back(p);
-    }
-  }
-  if (!fwd) passes.back().scale = true;  // 1/n folded into the last inverse pass
+  std::vector<PassDesc> passes = plan_ntt_passes(field, variant, dir, log_n, batch, layout, sc);
+  (void)fwd;
   // buffers: pass 0 reads the input; V3 alternates between out and scratch and
   // ends in out; the others work in place on out.
   const size_t bytes = total * sizeof(uint32_t);

@@ -105,6 +105,12 @@ class Context {
                      const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr);
 };
 
+std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
+                                      const Schedule &sc);
+sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src, uint32_t *dst,
+                          uint64_t src_cs, uint64_t src_rs, uint64_t dst_cs, uint64_t dst_rs, uint32_t batch,
+                          uint32_t log_n_real, PassArgs *out);
+
 sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t log_n, uint32_t batch,
                         int layout);
 sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout);

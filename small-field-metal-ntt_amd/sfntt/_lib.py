@@ -43,18 +43,18 @@ class Request(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("field", ctypes.c_int), ("dir", ctypes.c_int),
                 ("log_n", ctypes.c_uint32), ("batch", ctypes.c_uint32),
                 ("added_bits", ctypes.c_uint32), ("shift_raw", ctypes.c_uint32),
-                ("host_in", vp), ("host_out", vp)]
+                ("device_ptrs", ctypes.c_int), ("inp", vp), ("out", vp)]
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mem_budget_bytes", ctypes.c_size_t), ("max_inflight", ctypes.c_uint32),
-                ("streams", ctypes.c_uint32), ("max_queue", ctypes.c_uint32),
-                ("batching", ctypes.c_int)]
+                ("max_queue", ctypes.c_uint32), ("max_group", ctypes.c_uint32),
+                ("max_group_cols", ctypes.c_uint32)]
 
 
 class JobTiming(ctypes.Structure):
     _fields_ = [("submit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
-                ("end_ns", ctypes.c_uint64), ("status", ctypes.c_int)]
+                ("end_ns", ctypes.c_uint64), ("group_size", ctypes.c_uint32), ("status", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/sfntt.h declares
@@ -96,6 +96,9 @@ SIGNATURES = {
     "sfn_tuning_save": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t)]),
     "sfn_tuning_load": (ctypes.c_int, [vp, ctypes.c_char_p]),
+    "sfn_host_alloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+    "sfn_host_free": (ctypes.c_int, [vp, vp]),
+    "sfn_now": (ctypes.c_uint64, []),
     "sfn_server_create": (ctypes.c_int, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     "sfn_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request), u64p]),
     "sfn_server_poll": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(JobTiming)]),

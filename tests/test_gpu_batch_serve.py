@@ -1,0 +1,203 @@
+"""GPU tests of the multi-job plan (uneven sizes, both fields, one launch per
+pass, hipGraph replay), the autotuner, and the serving loop (every result the
+same alone and in the mix).  Expected values come from the CPU oracle, which
+restates each job's single-column reference transform."""
+import ctypes
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sfntt
+from sfntt import _lib as L
+from sfntt import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return sfntt.HipContext(0)
+
+
+def uneven_jobs(seed, count, lo, hi):
+    """Config-4 style draw: field ~ Bernoulli(1/2), log_n ~ U{lo..hi}, per-column LCG seed."""
+    st = orc.splitmix64_stream(seed)
+    jobs = []
+    for _ in range(count):
+        f = next(st) & 1
+        log_n = lo + next(st) % (hi - lo + 1)
+        s = next(st)
+        jobs.append((int(f), int(log_n), int(s)))
+    return jobs
+
+
+def run_plan(ctx, specs, use_graph, dirs=None, reps=1):
+    bufs, descs, inputs = [], [], []
+    for k, (f, log_n, s) in enumerate(specs):
+        x = synth.field_data(f, s, 1 << log_n)
+        b = sfntt.DeviceBuffer(ctx, x.size)
+        b.upload(x)
+        bufs.append(b)
+        inputs.append(x)
+        d = L.JobDesc(f, dirs[k] if dirs else L.FORWARD, log_n, 1, b.ptr, b.ptr)
+        descs.append(d)
+    arr = (L.JobDesc * len(descs))(*descs)
+    plan = ctypes.c_void_p()
+    ctx.check(ctx._lib.sfn_plan_create(ctx.ptr, arr, len(descs), int(use_graph), ctypes.byref(plan)))
+    nl, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    ctx.check(ctx._lib.sfn_plan_info(plan, ctypes.byref(nl), ctypes.byref(nt)))
+    for _ in range(reps):
+        ctx.check(ctx._lib.sfn_plan_launch(plan, None))
+    ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+    outs = [b.download() for b in bufs]
+    ctx._lib.sfn_plan_destroy(plan)
+    return inputs, outs, nl.value
+
+
+@pytest.mark.parametrize("use_graph", [0, 1], ids=["direct", "graph"])
+def test_uneven_mixed_plan_matches_single_columns(ctx, use_graph):
+    specs = uneven_jobs(0x5EED0004, 48, 10, 18)
+    dirs = [k % 2 for k in range(len(specs))]
+    inputs, outs, launches = run_plan(ctx, specs, use_graph, dirs)
+    assert launches <= 3  # one launch per pass, not per job
+    for (f, log_n, s), d, x, y in zip(specs, dirs, inputs, outs):
+        fn = {(0, 0): orc.m31_forward, (0, 1): orc.m31_inverse,
+              (1, 0): orc.bb_forward, (1, 1): orc.bb_inverse}[(f, d)]
+        assert np.array_equal(y, fn(x)), (f, log_n, d)
+
+
+def test_graph_replays_each_match(ctx):
+    # replay several times with host work between replays; every replay is a
+    # fresh forward of the previous output, compared with the oracle each time
+    specs = uneven_jobs(77, 12, 10, 16)
+    bufs, descs, cur = [], [], []
+    for f, log_n, s in specs:
+        x = synth.field_data(f, s, 1 << log_n)
+        b = sfntt.DeviceBuffer(ctx, x.size)
+        b.upload(x)
+        bufs.append(b)
+        cur.append(x)
+        descs.append(L.JobDesc(f, L.FORWARD, log_n, 1, b.ptr, b.ptr))
+    arr = (L.JobDesc * len(descs))(*descs)
+    plan = ctypes.c_void_p()
+    ctx.check(ctx._lib.sfn_plan_create(ctx.ptr, arr, len(descs), 1, ctypes.byref(plan)))
+    for rep in range(3):
+        ctx.check(ctx._lib.sfn_plan_launch(plan, None))
+        ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+        for k, ((f, log_n, s), b) in enumerate(zip(specs, bufs)):
+            exp = orc.m31_forward(cur[k]) if f == 0 else orc.bb_forward(cur[k])
+            got = b.download()
+            assert np.array_equal(got, exp), (rep, k)
+            cur[k] = exp
+    ctx._lib.sfn_plan_destroy(plan)
+
+
+def test_config4_full_size_plan(ctx):
+    # 512 columns, log_n ~ U{10..22}, both fields (the workload as stated)
+    specs = uneven_jobs(0x5EED0004, 512, 10, 22)
+    inputs, outs, launches = run_plan(ctx, specs, 1)
+    rng = np.random.default_rng(4)
+    check = rng.choice(len(specs), 40, replace=False)
+    for k in check:
+        f, log_n, s = specs[k]
+        exp = orc.m31_forward(inputs[k]) if f == 0 else orc.bb_forward(inputs[k])
+        assert np.array_equal(outs[k], exp), (f, log_n)
+
+
+def test_autotune_and_tuning_roundtrip(ctx):
+    best = ctypes.c_uint32()
+    ctx.check(ctx._lib.sfn_autotune(ctx.ptr, L.FIELD_BABYBEAR, 16, 4, L.LAYOUT_SOA, ctypes.byref(best)))
+    assert 6 <= best.value <= 11
+    n = ctypes.c_size_t()
+    ctx.check(ctx._lib.sfn_tuning_save(ctx.ptr, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    ctx.check(ctx._lib.sfn_tuning_save(ctx.ptr, buf, n.value, ctypes.byref(n)))
+    assert b'"log_n":16' in buf.value
+    ctx.check(ctx._lib.sfn_tuning_load(ctx.ptr, buf.value))
+    # the tuned plan still gives the reference result
+    x = orc.lcg_fill(5, 4 << 16, orc.FIELD_BB)
+    y, _ = ctx.ntt(L.FIELD_BABYBEAR, L.VARIANT_AUTO, L.FORWARD, 16, 4, x)
+    assert np.array_equal(y, orc.ntt_batch(1, False, 16, 4, x))
+    with pytest.raises(sfntt.api.InvalidArgument):
+        ctx.check(ctx._lib.sfn_tuning_load(ctx.ptr, b'[{"field":1}]'))
+
+
+# ------------------------------------------------------------------ serving
+def make_server(ctx, **kw):
+    cfg = L.ServerConfig(kw.get("mem", 0), kw.get("inflight", 4), kw.get("queue", 0),
+                         kw.get("group", 0), kw.get("cols", 0))
+    srv = ctypes.c_void_p()
+    ctx.check(ctx._lib.sfn_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(srv)))
+    return srv
+
+
+def mixed_requests(seed, count):
+    """Config-5 style mix (scaled down): 60% BabyBear LDE, 40% M31 NTT."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.6:
+            out.append(("lde", int(rng.integers(8, 13)), int(rng.integers(1, 9)), int(rng.integers(1, 3))))
+        else:
+            out.append(("ntt", int(rng.integers(10, 15)), 1, 0))
+    return out
+
+
+def expected(kind, log_n, batch, a, x):
+    if kind == "lde":
+        return orc.coset_lde_batch(x, log_n, batch, a)
+    return orc.m31_forward(x)
+
+
+@pytest.mark.parametrize("device_ptrs", [0, 1], ids=["host", "hbm"])
+def test_serving_results_same_alone_and_in_mix(ctx, device_ptrs):
+    reqs = mixed_requests(5, 60)
+    srv = make_server(ctx, inflight=4)
+    keep, tickets = [], []
+    for i, (kind, log_n, batch, a) in enumerate(reqs):
+        f = 1 if kind == "lde" else 0
+        x = synth.field_data(f, 1000 + i, batch << log_n)
+        nout = batch << (log_n + a)
+        if device_ptrs:
+            din, dout = sfntt.DeviceBuffer(ctx, x.size), sfntt.DeviceBuffer(ctx, nout)
+            din.upload(x)
+            inp, outp, out = din.ptr, dout.ptr, dout
+        else:
+            out = np.zeros(nout, np.uint32)
+            inp, outp = x.ctypes.data, out.ctypes.data
+            din = None
+        r = L.Request(L.JOB_LDE if kind == "lde" else L.JOB_NTT, f, L.FORWARD, log_n, batch, a, 0,
+                      device_ptrs, inp, outp)
+        t = ctypes.c_uint64()
+        ctx.check(ctx._lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+        keep.append((kind, log_n, batch, a, x, out, din))
+        tickets.append(t.value)
+    groups = []
+    for t in tickets:
+        tm = L.JobTiming()
+        ctx.check(ctx._lib.sfn_server_wait(srv, t, ctypes.byref(tm)))
+        assert tm.submit_ns <= tm.start_ns <= tm.end_ns
+        groups.append(tm.group_size)
+    ctx._lib.sfn_server_destroy(srv)
+    for kind, log_n, batch, a, x, out, din in keep:
+        got = out.download() if device_ptrs else out
+        assert np.array_equal(got, expected(kind, log_n, batch, a, x)), (kind, log_n, batch, a)
+
+
+def test_serving_admission_and_errors(ctx):
+    srv = make_server(ctx, inflight=1, mem=1 << 20, queue=2)
+    big = np.zeros(1 << 20, np.uint32)
+    out = np.zeros(1 << 20, np.uint32)
+    t = ctypes.c_uint64()
+    r = L.Request(L.JOB_NTT, 0, 0, 20, 1, 0, 0, 0, big.ctypes.data, out.ctypes.data)
+    # 8 MiB footprint never fits a 1 MiB budget: refused at submit
+    assert ctx._lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)) == L.ERR_BUFFER_ALLOC_FAILED
+    bad = L.Request(L.JOB_LDE, 0, 0, 10, 1, 1, 0, 0, big.ctypes.data, out.ctypes.data)
+    assert ctx._lib.sfn_server_submit(srv, ctypes.byref(bad), ctypes.byref(t)) == L.ERR_UNSUPPORTED
+    bad2 = L.Request(L.JOB_LDE, 1, 0, 26, 1, 1, 0, 0, big.ctypes.data, out.ctypes.data)
+    assert ctx._lib.sfn_server_submit(srv, ctypes.byref(bad2), ctypes.byref(t)) == L.ERR_INVALID_SIZE
+    assert ctx._lib.sfn_server_poll(srv, 12345, None) == L.ERR_INVALID_ARGUMENT
+    ctx._lib.sfn_server_destroy(srv)
