@@ -453,25 +453,27 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
   timing_begin_sequence(s);
   Schedule sc = force ? *force : schedule_for(2 /*lde*/, log_n, batch, layout);
-  // middle pass covers [lo, log_n); K1 and K3 cover [0, lo)
-  uint32_t mid_t = log_n <= sc.max_log_t ? log_n : (log_n + 1) / 2;
-  if (mid_t > sc.max_log_t) mid_t = sc.max_log_t;
-  if (log_n - mid_t > sc.max_log_t) return fail(SFN_ERR_UNSUPPORTED, "LDE needs log_n <= 2 * max tile");
+  // middle pass covers [lo, log_n); K1 (inverse) and K3 (forward) cover [0, lo),
+  // each in one or more passes of at most max_log_t layers
+  uint32_t mlt = std::min<uint32_t>(sc.max_log_t, kMaxLogTile);
+  uint32_t mid_t = log_n <= mlt ? log_n : (log_n + 1) / 2;
+  if (mid_t > mlt) mid_t = mlt;
   const uint32_t lo = log_n - mid_t;
   const Layout Lin = layout_of(layout, log_n, batch);
   const Layout Lout = layout_of(layout, LN, batch);
   sfn_status st;
+  auto outer = split_layers(lo, mlt);  // ascending (lo', log_t')
   // K1: inverse layers [0, lo) (no normalisation) into the first n rows of out
   const uint32_t *mid_src = d_in;
   Layout mid_ls = Lin;
-  if (lo > 0) {
+  for (size_t i = 0; i < outer.size(); ++i) {
     PassDesc p;
-    p.lo = 0;
-    p.log_t = (int)lo;
-    p.logr = (int)std::min<uint32_t>(sc.logr, lo);
+    p.lo = outer[i].first;
+    p.log_t = (int)outer[i].second;
+    p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = log_n;
     choose_lanes(p, layout, batch, sc.log_w);
-    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, d_in, d_out, Lin, Lout, batch, log_n, s);
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, d_out, mid_ls, Lout, batch, log_n, s);
     if (st) return st;
     mid_src = d_out;
     mid_ls = Lout;
@@ -522,12 +524,12 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
     (void)timing_mark(s, false, "");
   }
-  // K3: forward layers [0, lo) on every output block (the N-row array)
-  if (lo > 0) {
+  // K3: forward layers [lo-1 .. 0] on every output block (the N-row array)
+  for (size_t i = outer.size(); i-- > 0;) {
     PassDesc p;
-    p.lo = 0;
-    p.log_t = (int)lo;
-    p.logr = (int)std::min<uint32_t>(sc.logr, lo);
+    p.lo = outer[i].first;
+    p.log_t = (int)outer[i].second;
+    p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = LN;
     choose_lanes(p, layout, batch, sc.log_w);
     st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s);
