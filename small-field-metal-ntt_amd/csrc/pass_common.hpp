@@ -1,0 +1,152 @@
+// Pieces shared by the pass kernels: round schedule, lane decoding, register
+// rounds and twiddle fetchers.
+#pragma once
+#include "field.hpp"
+#include "kernels.hpp"
+
+namespace sfn {
+
+template <int LOG_T>
+struct Tile {
+  static constexpr int T = 1 << LOG_T;
+  static constexpr int TP = T + (T >> 5);  // lane pitch: one pad word per 32 points
+  __device__ static __forceinline__ int idx(int p) { return p + (p >> 5); }
+};
+
+// Round schedule: forward covers bits top-down in chunks of LOGR with the
+// partial chunk last (lowest bits); inverse covers bits bottom-up with the
+// partial chunk first, so the inverse's last round and the forward's first
+// round own the same points.
+template <int LOG_T, int LOGR>
+struct Sched {
+  static constexpr int NR = (LOG_T + LOGR - 1) / LOGR;
+  static constexpr int REM = LOG_T - (NR - 1) * LOGR;
+  template <int DIR, int r>
+  static constexpr int k() {
+    return DIR == 0 ? (r == NR - 1 ? REM : LOGR) : (r == 0 ? REM : LOGR);
+  }
+  template <int DIR, int r>
+  static constexpr int b0() {
+    return DIR == 0 ? (r == NR - 1 ? 0 : LOG_T - (r + 1) * LOGR) : (r == 0 ? 0 : REM + (r - 1) * LOGR);
+  }
+};
+
+struct LaneInfo {
+  uint64_t sbase, dbase;
+  uint64_t hb;
+  uint32_t low, valid;
+};
+
+__device__ __forceinline__ void decode_lane(const PassArgs &a, int log_t, uint64_t g, LaneInfo &li) {
+  const uint32_t lo = a.lo;
+  const uint32_t hi = lo + log_t;
+  const uint32_t lo_mask = (1u << lo) - 1;
+  uint32_t c, low;
+  uint64_t hb;
+  li.valid = g < a.instances;
+  if (!li.valid) g = 0;
+  if (a.order == kLowFastest) {
+    low = (uint32_t)g & lo_mask;
+    uint64_t r = g >> lo;
+    c = (uint32_t)(r % a.batch);
+    hb = r / a.batch;
+  } else if (a.order == kColFastest) {
+    c = (uint32_t)(g % a.batch);
+    uint64_t r = g / a.batch;
+    low = (uint32_t)r & lo_mask;
+    hb = r >> lo;
+  } else {
+    const uint32_t nhb_log = a.log_n - hi;
+    hb = g & ((1ull << nhb_log) - 1);
+    uint64_t r = g >> nhb_log;
+    low = 0;
+    c = (uint32_t)r;
+  }
+  const uint64_t row0 = (hb << hi) | low;
+  li.sbase = (uint64_t)c * a.src_cs + row0 * a.src_rs;
+  li.dbase = (uint64_t)c * a.dst_cs + row0 * a.dst_rs;
+  li.hb = hb;
+  li.low = low;
+}
+
+// ---- twiddle fetch inside a register round ----
+// BabyBear: small position table staged in LDS (layer t at offset 2^t - 1).
+// M31: block table in global memory, index (hb << (hi-l-1)) | (p >> (t+1)).
+template <class F>
+struct TwCtx;
+
+template <>
+struct TwCtx<BabyBear> {
+  const uint2 *stw;  // LDS
+  __device__ __forceinline__ uint2 get(int t, uint32_t pos) const { return stw[(1u << t) - 1 + pos]; }
+};
+
+template <>
+struct TwCtx<M31> {
+  const uint32_t *tbl;  // global, whole transform
+  uint32_t n_log, lo, hi;
+  uint64_t hb;
+  // layer t of the sub-transform (global layer lo + t), block index inside the sub-transform
+  __device__ __forceinline__ uint32_t get(int t, uint32_t blk) const {
+    const uint32_t l = lo + t;
+    const uint64_t off = (1ull << n_log) - (1ull << (n_log - l));
+    return __ldg(tbl + off + ((hb << (hi - l - 1)) | blk));
+  }
+};
+
+// One register round: K layers on v[0..2^K) holding points pbase | (i << B0).
+template <class F, int DIR, int K, int B0, class TW>
+__device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase, const TW &tw) {
+  constexpr int R = 1 << K;
+  constexpr bool GS = (F::ID == 1) == (DIR == 0);  // BB fwd and M31 inv are GS
+  if constexpr (DIR == 0) {
+#pragma unroll
+    for (int j = K - 1; j >= 0; --j) {
+#pragma unroll
+      for (int m = 0; m < (1 << (F::ID == 1 ? j : (K - 1 - j))); ++m) {
+        typename F::Tw w;
+        if constexpr (F::ID == 1) {
+          w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
+        } else {
+          w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
+        }
+#pragma unroll
+        for (int q = 0; q < (1 << (F::ID == 1 ? (K - 1 - j) : j)); ++q) {
+          // BB: twiddle depends on i mod 2^j -> i = m + q*2^(j+1)
+          // M31: twiddle depends on i >> (j+1) -> i = (m << (j+1)) + q
+          const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          else bfly_ct<F>(v[i], v[i + (1 << j)], w);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+#pragma unroll
+      for (int m = 0; m < (1 << (F::ID == 1 ? j : (K - 1 - j))); ++m) {
+        typename F::Tw w;
+        if constexpr (F::ID == 1) {
+          w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
+        } else {
+          w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
+        }
+#pragma unroll
+        for (int q = 0; q < (1 << (F::ID == 1 ? (K - 1 - j) : j)); ++q) {
+          const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          else bfly_ct<F>(v[i], v[i + (1 << j)], w);
+        }
+      }
+    }
+  }
+  (void)R;
+}
+
+template <int K, int B0>
+__device__ __forceinline__ uint32_t round_pbase(uint32_t rb) {
+  return (rb & ((1u << B0) - 1)) | ((rb >> B0) << (B0 + K));
+}
+
+
+}  // namespace sfn

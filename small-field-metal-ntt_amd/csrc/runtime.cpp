@@ -39,6 +39,7 @@ Context::~Context() {
   rel(bb_pos[0]);
   rel(bb_pos[1]);
   rel(bb_small_pair);
+  rel(bb_root);
   for (auto &kv : bb_twist) rel(kv.second);
   for (auto &kv : bb_lde_mid) rel(kv.second);
   for (auto &kv : m31_tw) rel(kv.second);
@@ -126,6 +127,17 @@ sfn_status Context::bb_small_pair_table(const uint2 **out) {
     if (s) return s;
   }
   *out = (const uint2 *)bb_small_pair.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_root_table(const uint2 **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  if (!bb_root.ptr) {
+    auto t = host::bb_root_table(kMaxLogTile);
+    sfn_status s = upload_table(bb_root, t.data(), t.size() * sizeof(host::Pair));
+    if (s) return s;
+  }
+  *out = (const uint2 *)bb_root.ptr;
   return SFN_OK;
 }
 
@@ -267,10 +279,37 @@ void choose_lanes(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) 
   p.tiles = (instances + (1ull << lw) - 1) >> lw;
 }
 
+// Lane geometry of a tuned pass: blockDim = W << (log_t - 5) threads, one item
+// per thread in the full rounds; LDS within the per-workgroup limit.
+bool choose_lanes_fast(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) {
+  if (p.log_t < 6 || p.log_t > (int)kMaxLogTile) return false;
+  uint32_t lw;
+  if (layout == SFN_LAYOUT_ROW_MAJOR) {
+    p.order = kColFastest;
+    p.lane_contig = 1;
+    lw = want_log_w;
+  } else if (p.lo > 0) {
+    p.order = kLowFastest;
+    p.lane_contig = 1;
+    lw = std::min<uint32_t>(want_log_w, p.lo);
+  } else {
+    p.lane_contig = 0;
+    p.order = batch >= 4 ? kColFastest : kHighFastest;
+    lw = p.log_t >= 13 ? 0 : 13 - p.log_t;  // 256 threads
+  }
+  while (lw > 0 && (((1u << lw) << (p.log_t - 5)) > 512 || fast_lds_bytes(1, p.log_t, lw) > kMaxLdsBytes)) --lw;
+  if (fast_lds_bytes(1, p.log_t, lw) > kMaxLdsBytes) return false;
+  p.log_w = lw;
+  p.fast = true;
+  const uint64_t instances = (uint64_t)batch << (p.log_n - p.log_t);
+  p.tiles = (instances + (1ull << lw) - 1) >> lw;
+  return true;
+}
+
 }  // namespace
 
 std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
-                                      const Schedule &sc) {
+                                      const Schedule &sc, bool allow_fast) {
   (void)field;
   std::vector<PassDesc> passes;
   const bool fwd = dir == SFN_FORWARD;
@@ -296,7 +335,8 @@ std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t 
       p.log_t = (int)pr.second;
       p.logr = (int)std::min<uint32_t>(logr, pr.second);
       p.log_n = log_n;
-      choose_lanes(p, layout, batch, sc.log_w);
+      if (!(allow_fast && logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w)))
+        choose_lanes(p, layout, batch, sc.log_w);
       passes.push_back(p);
     }
   }
@@ -400,6 +440,28 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   PassArgs a{};
   sfn_status st0 = make_pass_args(ctx, field, dir, p, src, dst, ls.cs, ls.rs, ld.cs, ld.rs, batch, log_n_real, &a);
   if (st0) return st0;
+  // the point-adjacent tuned kernel moves 16-byte vectors: needs aligned buffers
+  const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
+  if (p.fast && (p.lane_contig || aligned)) {
+    if (field == SFN_FIELD_BABYBEAR) {
+      const uint2 *root;
+      if ((st0 = ctx->bb_root_table(&root))) return st0;
+      a.tw = root;
+    }
+    (void)ctx->timing_mark(s, true, std::string("k_fast<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
+                                       std::to_string(p.log_t) + (p.lane_contig ? ",lanes" : ",points") +
+                                       ">lo=" + std::to_string(p.lo));
+    hipError_t e = launch_fast(field, dir, p.log_t, p.lane_contig != 0, a, p.tiles, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "tuned pass launch");
+    (void)ctx->timing_mark(s, false, "");
+    return SFN_OK;
+  }
+  if (p.fast) {  // fall back to the generic kernel with its own geometry
+    PassDesc q = p;
+    q.fast = false;
+    choose_lanes(q, SFN_LAYOUT_SOA, batch, 4);
+    return launch_desc(ctx, field, dir, q, src, dst, ls, ld, batch, log_n_real, s);
+  }
   (void)ctx->timing_mark(s, true, std::string("k_pass<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
                                      std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
                                      std::to_string(p.lo));
@@ -422,7 +484,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
   timing_begin_sequence(s);
   Schedule sc = force ? *force : schedule_for(field, log_n, batch, layout);
   const bool fwd = dir == SFN_FORWARD;
-  std::vector<PassDesc> passes = plan_ntt_passes(field, variant, dir, log_n, batch, layout, sc);
+  std::vector<PassDesc> passes = plan_ntt_passes(field, variant, dir, log_n, batch, layout, sc, true);
   (void)fwd;
   // buffers: pass 0 reads the input; V3 alternates between out and scratch and
   // ends in out; the others work in place on out.
@@ -472,7 +534,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.log_t = (int)outer[i].second;
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = log_n;
-    choose_lanes(p, layout, batch, sc.log_w);
+    if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
     st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, d_out, mid_ls, Lout, batch, log_n, s);
     if (st) return st;
     mid_src = d_out;
@@ -508,21 +570,39 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     a.lane_contig = p.lane_contig;
     a.instances = (uint64_t)batch << lo;
     const uint2 *pair, *twi, *twf, *mid;
-    if ((st = bb_small_pair_table(&pair))) return st;
     if ((st = bb_twist_table(lo, log_n, true, &twi))) return st;
     if ((st = bb_twist_table(lo, log_n, false, &twf))) return st;
     if ((st = bb_mid_table(log_n, added_bits, shift_std, &mid))) return st;
-    a.tw = pair;
     a.twist = twi;
     a.twist_fwd = twf;
     a.mid = mid;
     a.blocks = 1u << added_bits;
     a.dst_block_off = ((uint64_t)1 << log_n) * Lout.rs;
-    const uint64_t tiles = (a.instances + (1ull << lw) - 1) >> lw;
-    (void)timing_mark(s, true, "k_lde_mid<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
-    hipError_t e = launch_lde_mid((int)mid_t, a, tiles, s);
-    if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
-    (void)timing_mark(s, false, "");
+    // tuned middle kernel: lanes adjacent in memory, one item per thread
+    PassDesc q;
+    q.lo = lo;
+    q.log_t = (int)mid_t;
+    q.log_n = log_n;
+    if (sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig) {
+      const uint2 *root;
+      if ((st = bb_root_table(&root))) return st;
+      a.tw = root;
+      a.log_w = q.log_w;
+      a.order = q.order;
+      a.lane_contig = 1;
+      (void)timing_mark(s, true, "k_mid_fast<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
+      hipError_t e = launch_lde_mid_fast((int)mid_t, a, q.tiles, s);
+      if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
+      (void)timing_mark(s, false, "");
+    } else {
+      if ((st = bb_small_pair_table(&pair))) return st;
+      a.tw = pair;
+      const uint64_t tiles = (a.instances + (1ull << lw) - 1) >> lw;
+      (void)timing_mark(s, true, "k_lde_mid<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
+      hipError_t e = launch_lde_mid((int)mid_t, a, tiles, s);
+      if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
+      (void)timing_mark(s, false, "");
+    }
   }
   // K3: forward layers [lo-1 .. 0] on every output block (the N-row array)
   for (size_t i = outer.size(); i-- > 0;) {
@@ -531,7 +611,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.log_t = (int)outer[i].second;
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = LN;
-    choose_lanes(p, layout, batch, sc.log_w);
+    if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
     st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s);
     if (st) return st;
   }

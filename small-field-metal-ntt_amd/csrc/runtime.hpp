@@ -24,6 +24,7 @@ struct PassDesc {
   bool layer_kernel = false;  // V1: one butterfly per thread for `layer`
   uint32_t layer = 0;
   bool lde_mid = false;
+  bool fast = false;          // tuned kernel (fast.hip): radix-32, one item per thread
   int log_t = 0, logr = 0;
   uint32_t lo = 0;
   uint32_t log_n = 0;         // rows of the (virtual) transform the pass indexes
@@ -38,7 +39,7 @@ struct Schedule {
   uint32_t max_log_t = 10;  // largest LDS sub-transform
   uint32_t logr = 5;        // register radix
   uint32_t log_w = 4;       // lanes per workgroup (strided passes)
-  uint32_t mid_log_w = 3;   // lanes per workgroup of the LDE middle pass
+  uint32_t mid_log_w = 4;   // lanes per workgroup of the LDE middle pass
 };
 
 struct TuneKey {
@@ -64,6 +65,7 @@ class Context {
   DevBuf bb_pos[2];     // position tables (fwd, inv), size 2^bb_pos_log pairs
   uint32_t bb_pos_log = 0;
   DevBuf bb_small_pair; // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
+  DevBuf bb_root;       // g_{2^11}^e, e < 2^11, Shoup pairs (tuned kernels)
   std::map<std::tuple<uint32_t, uint32_t, int>, DevBuf> bb_twist;         // (lo, hi, inverse)
   std::map<std::tuple<uint32_t, uint32_t, uint32_t>, DevBuf> bb_lde_mid;  // (L, a, shift_std)
   std::map<std::pair<uint32_t, int>, DevBuf> m31_tw;                      // (L, inverse)
@@ -92,6 +94,7 @@ class Context {
   sfn_status upload_table(DevBuf &b, const void *host, size_t bytes);
   sfn_status bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);
+  sfn_status bb_root_table(const uint2 **out);
   sfn_status bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
   sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
   sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);
@@ -106,7 +109,7 @@ class Context {
 };
 
 std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
-                                      const Schedule &sc);
+                                      const Schedule &sc, bool allow_fast = false);
 sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src, uint32_t *dst,
                           uint64_t src_cs, uint64_t src_rs, uint64_t dst_cs, uint64_t dst_rs, uint32_t batch,
                           uint32_t log_n_real, PassArgs *out);

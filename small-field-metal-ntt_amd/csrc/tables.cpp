@@ -18,6 +18,17 @@ std::vector<Pair> bb_position_table(uint32_t log_max, bool inverse) {
   return t;
 }
 
+std::vector<Pair> bb_root_table(uint32_t log) {
+  std::vector<Pair> t((size_t)1 << log);
+  const uint32_t g = bb_root(log);
+  uint32_t w = 1;
+  for (size_t e = 0; e < t.size(); ++e) {
+    t[e] = shoup(w, kBBP);
+    w = mulmod(w, g, kBBP);
+  }
+  return t;
+}
+
 std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse) {
   const uint32_t lt = hi - lo;
   const uint64_t mask = ((uint64_t)1 << hi) - 1;

@@ -55,6 +55,10 @@ inline uint64_t bitrev(uint64_t v, uint32_t bits) {
 // g_{s+1}^(+-j), j < 2^s, as Shoup pairs.  Independent of the transform size.
 std::vector<Pair> bb_position_table(uint32_t log_max, bool inverse);
 
+// Root table g_{2^log}^e, e < 2^log (Shoup pairs): one table for every layer
+// and both directions of a sub-transform of up to 2^log points.
+std::vector<Pair> bb_root_table(uint32_t log);
+
 // BabyBear four-step twist of layers [lo, hi): entry (p << lo) | low holds
 // g_hi^(+-low * bitrev(p, hi - lo)).
 std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse);
