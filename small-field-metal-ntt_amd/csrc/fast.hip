@@ -5,14 +5,20 @@
 //    the last round straight from registers, so a 2^10-point pass touches LDS
 //    once per element (one exchange) and each thread has 32 independent loads
 //    in flight;
+//  - address HBM through buffer resources rebased per workgroup: one 32-bit
+//    VGPR offset per item, the per-point stride in a scalar offset, so the 32
+//    loads and 32 stores of an item cost no vector ALU work for addresses;
 //  - run exactly one item (one lane x 32 points) per thread in the full
 //    rounds (blockDim = W * 2^(LOG_T-5));
 //  - keep one root table g_T^e (e < T) in LDS that serves every layer and both
 //    directions (layer t, position j uses e = j * 2^(LOG_T-t-1), negated for
 //    the inverse);
+//  - stream the four-step twists as single-word Montgomery constants;
 //  - lay LDS out so the exchanges are bank-conflict free: lane-fastest
 //    (p + p/32) * W + lane when lanes are adjacent in memory, lane-major
 //    lane * (T + T/32) + p + p/32 when points are.
+// The host only selects these kernels when every offset inside a workgroup
+// fits a 32-bit buffer offset.
 #include "pass_common.hpp"
 
 namespace sfn {
@@ -25,8 +31,22 @@ namespace sfn {
 namespace fast {
 
 constexpr int KR = 5;
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
 
-// BabyBear twiddles from the LDS root table.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void *base, uint64_t bytes) {
+  const uint32_t n = bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)n, kRsrcWord3);
+}
+__device__ __forceinline__ uint32_t bload(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstore(uint32_t v, Rsrc r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
+}
+
+// BabyBear twiddles from the LDS root table (Shoup pairs).
 template <int LOG_T, int DIR>
 struct RootTw {
   const uint2 *tab;
@@ -52,15 +72,18 @@ __device__ __forceinline__ uint32_t lidx(uint32_t lane, uint32_t p, uint32_t log
   else return lane * (uint32_t)((1 << LOG_T) + (1 << LOG_T) / 32) + p + (p >> 5);
 }
 
-// Per-launch state shared by the rounds.
+// Per-workgroup state shared by the rounds.  All offsets are relative to the
+// workgroup's first lane (lane 0 has the lowest address of the tile).
 struct RoundIO {
-  const uint32_t *src;
-  uint32_t *dst;
-  uint64_t sps, dps;    // element stride between consecutive points
-  uint64_t dst_off;     // extra offset added to every store (LDE output block)
-  const uint2 *twist_in, *twist_out;  // BabyBear four-step twists (or null)
+  Rsrc src, dst, twist;
+  uint64_t sbase0, dbase0;  // element index of lane 0's row 0 in src / dst
+  uint32_t sps, dps;        // elements between consecutive points
   uint32_t lo;
+  bool has_twist;
   uint32_t has_scale, scale_w, scale_q;
+  // LDE blocks: per-point factor table (one block) and post-twist table (one block)
+  Rsrc pointf, post;
+  bool has_pointf, has_post;
 };
 
 // Round r of a pass: loads from HBM when FIRST, else from LDS; stores to HBM
@@ -81,23 +104,25 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     uint32_t v[R];
     if constexpr (FIRST) {
       if (li.valid) {
+        const uint32_t voff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pbase * io.sps) * 4u;
         if constexpr (!CONTIG && B0 == 0 && K >= 2) {
           // R consecutive words per thread (points are adjacent rows)
-          const uint4 *s4 = reinterpret_cast<const uint4 *>(io.src + li.sbase + pbase);
 #pragma unroll
           for (int i = 0; i < R / 4; ++i) {
-            const uint4 q = s4[i];
+            const uint4 q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(io.src, (int)(voff + 16 * i), 0, 0));
             v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
           }
         } else {
+          const uint32_t step = (io.sps << B0) * 4u;
 #pragma unroll
-          for (int i = 0; i < R; ++i) v[i] = io.src[li.sbase + (uint64_t)(pbase | (i << B0)) * io.sps];
+          for (int i = 0; i < R; ++i) v[i] = bload(io.src, voff, i * step);
         }
         if constexpr (F::ID == 1 && DIR == 1) {
-          if (io.twist_in) {
+          if (io.has_twist) {
+            const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
+            const uint32_t tstep = (4u << (B0 + io.lo));
 #pragma unroll
-            for (int i = 0; i < R; ++i)
-              v[i] = BabyBear::mul_tw(v[i], io.twist_in[((uint64_t)(pbase | (i << B0)) << io.lo) | li.low]);
+            for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
           }
         }
       } else {
@@ -113,10 +138,17 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     if constexpr (LAST) {
       if (!li.valid) continue;
       if constexpr (F::ID == 1 && DIR == 0) {
-        if (io.twist_out) {
+        if (io.has_twist) {
+          const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
+          const uint32_t tstep = (4u << (B0 + io.lo));
 #pragma unroll
-          for (int i = 0; i < R; ++i)
-            v[i] = BabyBear::mul_tw(v[i], io.twist_out[((uint64_t)(pbase | (i << B0)) << io.lo) | li.low]);
+          for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
+        }
+        if (io.has_post) {
+          const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
+          const uint32_t tstep = (4u << (B0 + io.lo));
+#pragma unroll
+          for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.post, toff, i * tstep));
         }
       }
       if (io.has_scale) {
@@ -126,13 +158,18 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
           else v[i] = M31::mul(v[i], io.scale_w);
         }
       }
+      const uint32_t voff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pbase * io.dps) * 4u;
       if constexpr (!CONTIG && B0 == 0 && K >= 2) {
-        uint4 *d4 = reinterpret_cast<uint4 *>(io.dst + io.dst_off + li.dbase + pbase);
 #pragma unroll
-        for (int i = 0; i < R / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+        for (int i = 0; i < R / 4; ++i)
+          __builtin_amdgcn_raw_buffer_store_b128(
+              __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int,
+                                 make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3])),
+              io.dst, (int)(voff + 16 * i), 0, 0);
       } else {
+        const uint32_t step = (io.dps << B0) * 4u;
 #pragma unroll
-        for (int i = 0; i < R; ++i) io.dst[io.dst_off + li.dbase + (uint64_t)(pbase | (i << B0)) * io.dps] = v[i];
+        for (int i = 0; i < R; ++i) bstore(v[i], io.dst, voff, i * step);
       }
     } else {
 #pragma unroll
@@ -182,6 +219,30 @@ __device__ __forceinline__ void load_root(const PassArgs &a, uint2 *tab) {
   for (uint32_t e = threadIdx.x; e < (1u << LOG_T); e += blockDim.x) tab[e] = root[e << (kMaxLogTile - LOG_T)];
 }
 
+// Common workgroup set-up: lane records in LDS, buffer resources rebased at
+// lane 0 (decoded by every thread, so the bases stay scalar).
+template <int LOG_T>
+__device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundIO &io) {
+  const uint32_t log_w = a.log_w;
+  const uint64_t g0 = (uint64_t)blockIdx.x << log_w;
+  for (uint32_t l = threadIdx.x; l < (1u << log_w); l += blockDim.x) decode_lane(a, LOG_T, g0 + l, lanes[l]);
+  LaneInfo l0;
+  decode_lane(a, LOG_T, g0, l0);
+  io.sbase0 = l0.sbase;
+  io.dbase0 = l0.dbase;
+  io.src = make_rsrc(a.src + l0.sbase, (a.src_elems - l0.sbase) * 4);
+  io.dst = make_rsrc(a.dst + l0.dbase, (a.dst_elems - l0.dbase) * 4);
+  io.sps = (uint32_t)(((uint64_t)1 << a.lo) * a.src_rs);
+  io.dps = (uint32_t)(((uint64_t)1 << a.lo) * a.dst_rs);
+  io.lo = a.lo;
+  io.has_twist = a.twist_m != nullptr;
+  if (io.has_twist) io.twist = make_rsrc(a.twist_m, (uint64_t)4 << (a.lo + LOG_T));
+  io.has_scale = a.has_scale;
+  io.scale_w = a.scale_w;
+  io.scale_q = a.scale_q;
+  io.has_pointf = io.has_post = false;
+}
+
 template <class F, int DIR, int LOG_T, bool CONTIG>
 __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -189,29 +250,18 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
   uint32_t *tile = reinterpret_cast<uint32_t *>(tabp + (F::ID == 1 ? ((size_t)8 << LOG_T) : 0));
-  const uint64_t g0 = (uint64_t)blockIdx.x << log_w;
-  for (uint32_t l = threadIdx.x; l < (1u << log_w); l += blockDim.x) decode_lane(a, LOG_T, g0 + l, lanes[l]);
+  RoundIO io;
+  setup<LOG_T>(a, lanes, io);
   if constexpr (F::ID == 1) load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   __syncthreads();
-  RoundIO io;
-  io.src = a.src;
-  io.dst = a.dst;
-  io.sps = ((uint64_t)1 << a.lo) * a.src_rs;
-  io.dps = ((uint64_t)1 << a.lo) * a.dst_rs;
-  io.dst_off = 0;
-  io.twist_in = (F::ID == 1 && DIR == 1) ? a.twist : nullptr;
-  io.twist_out = (F::ID == 1 && DIR == 0) ? a.twist : nullptr;
-  io.lo = a.lo;
-  io.has_scale = a.has_scale;
-  io.scale_w = a.scale_w;
-  io.scale_q = a.scale_q;
   const auto tw = make_tw<F, DIR, LOG_T>(a, tabp);
   rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
 }
 
 // ---- coset LDE middle pass ----
-// inverse layers [lo, log_n) (pre-twist on load), x n^-1 (shift*zeta_b)^row,
-// forward layers [log_n-1, lo] (post-twist on store) for every output block.
+// inverse layers [lo, log_n) (pre-twist on load), then for every output block
+// b: x sigma_b^(p << lo), forward layers [log_n-1, lo], x post_b (forward
+// twist x n^-1 x sigma_b^low) on store, where sigma_b = shift * zeta_b.
 // The inverse's last round and the forward's first round own the same points
 // (one item per thread), so the coefficients stay in registers across blocks.
 template <int LOG_T>
@@ -229,22 +279,14 @@ __global__ __launch_bounds__(512) void k_mid_fast(const PassArgs a) {
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
   uint32_t *tile = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
-  const uint64_t g0 = (uint64_t)blockIdx.x << log_w;
-  for (uint32_t l = threadIdx.x; l < (1u << log_w); l += blockDim.x) decode_lane(a, LOG_T, g0 + l, lanes[l]);
+  RoundIO io;
+  setup<LOG_T>(a, lanes, io);
   load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   __syncthreads();
   RootTw<LOG_T, 1> twi;
   RootTw<LOG_T, 0> twf;
   twi.tab = twf.tab = reinterpret_cast<const uint2 *>(tabp);
-  RoundIO io{};
-  io.src = a.src;
-  io.dst = a.dst;
-  io.sps = ((uint64_t)1 << a.lo) * a.src_rs;
-  io.dps = ((uint64_t)1 << a.lo) * a.dst_rs;
-  io.twist_in = a.twist;
-  io.twist_out = a.twist_fwd;
-  io.lo = a.lo;
-  // inverse rounds 0 .. NR-2 (round 0 loads from HBM); they end with an LDS write
+  // inverse rounds 0 .. NR-2 (round 0 loads from HBM with the inverse twist)
   rounds<F, 1, LOG_T, true, 0, S::NR - 1>(tile, log_w, lanes, twi, io);
   __syncthreads();
   // inverse last round: one item per thread (blockDim == W << (LOG_T - K))
@@ -256,19 +298,21 @@ __global__ __launch_bounds__(512) void k_mid_fast(const PassArgs a) {
 #pragma unroll
   for (int i = 0; i < R; ++i) c[i] = tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)];
   round_regs<F, 1, K, B0>(c, pbase, twi);
-  const uint32_t low = lanes[lane].low;
   const uint64_t n = 1ull << a.log_n;
+  io.has_twist = false;  // the forward twist is part of post_b
+  io.has_post = true;
   for (uint32_t b = 0; b < a.blocks; ++b) {
-    const uint2 *mid = a.mid + b * n;
+    const Rsrc pf = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
     uint32_t v[R];
 #pragma unroll
-    for (int i = 0; i < R; ++i) v[i] = F::mul_tw(c[i], mid[((uint64_t)(pbase | (i << B0)) << a.lo) | low]);
+    for (int i = 0; i < R; ++i) v[i] = F::mul_mont(c[i], bload(pf, pbase * 4u, (uint32_t)(i << B0) * 4u));
     round_regs<F, 0, K, B0>(v, pbase, twf);
     __syncthreads();  // every thread is done reading the tile
 #pragma unroll
     for (int i = 0; i < R; ++i) tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)] = v[i];
     __syncthreads();
-    io.dst_off = b * a.dst_block_off;
+    io.dst = make_rsrc(a.dst + b * a.dst_block_off + io.dbase0, (a.dst_elems - b * a.dst_block_off - io.dbase0) * 4);
+    io.post = make_rsrc(a.post_m + b * n, n * 4);
     rounds<F, 0, LOG_T, true, 1, S::NR>(tile, log_w, lanes, twf, io);
   }
 }

@@ -62,6 +62,15 @@ struct BabyBear {
     uint32_t q = __umulhi(x, w.y);
     return x * w.x - mulp(q);
   }
+  // x * w for a twiddle held in Montgomery form wm = w * 2^32 mod p, any 32-bit
+  // x: t = x*wm < p*2^32, m = t_lo * p^-1 (mod 2^32) makes t - m*p divisible by
+  // 2^32 with no borrow, so (t - m*p) / 2^32 = t_hi - mulhi(m, p) in (-p, p).
+  __device__ static __forceinline__ uint32_t mul_mont(uint32_t x, uint32_t wm) {
+    const uint64_t t = (uint64_t)x * wm;
+    const uint32_t m = (uint32_t)t * 0x88000001u;  // p^-1 mod 2^32
+    const uint32_t r = (uint32_t)(t >> 32) - __umulhi(m, P);
+    return min(r, r + P);
+  }
   // Montgomery product of two raw values (src/field/babybear.rs:133-137).
   __device__ static __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
     uint64_t t = (uint64_t)a * b;

@@ -41,6 +41,11 @@ struct PassArgs {
   const uint2 *twist_fwd; // forward twist of the blocks' high pass
   uint32_t blocks;
   uint64_t dst_block_off; // element offset between consecutive output blocks
+  // tuned kernels (fast.hip): buffer extents and single-word Montgomery tables
+  uint64_t src_elems, dst_elems;  // elements addressable from src / dst
+  const uint32_t *twist_m;        // four-step twist of this pass, Montgomery form
+  const uint32_t *post_m;         // LDE: per-block post-twist x n^-1 * sigma_b^low (blocks x 2^log_n)
+  const uint32_t *pointf_m;       // LDE: per-block point factors sigma_b^(p << lo) (blocks x 2^LOG_T)
 };
 
 // V1 (one launch per layer) arguments.

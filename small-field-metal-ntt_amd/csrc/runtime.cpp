@@ -43,6 +43,11 @@ Context::~Context() {
   for (auto &kv : bb_twist) rel(kv.second);
   for (auto &kv : bb_lde_mid) rel(kv.second);
   for (auto &kv : m31_tw) rel(kv.second);
+  for (auto &kv : bb_twist_m) rel(kv.second);
+  for (auto &kv : bb_lde_fast_tab) {
+    rel(kv.second.first);
+    rel(kv.second.second);
+  }
   rel(scratch_in);
   rel(scratch_out);
   rel(scratch_pp);
@@ -138,6 +143,41 @@ sfn_status Context::bb_root_table(const uint2 **out) {
     if (s) return s;
   }
   *out = (const uint2 *)bb_root.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, const uint32_t **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto key = std::make_tuple(lo, hi, (int)inverse);
+  auto it = bb_twist_m.find(key);
+  if (it == bb_twist_m.end()) {
+    auto t = host::to_monty_words(host::bb_twist_table(lo, hi, inverse));
+    DevBuf b;
+    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
+    if (s) return s;
+    it = bb_twist_m.emplace(key, b).first;
+  }
+  *out = (const uint32_t *)it->second.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
+                                const uint32_t **post, const uint32_t **pointf) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto key = std::make_tuple(log_n, added_bits, shift_std, lo);
+  auto it = bb_lde_fast_tab.find(key);
+  if (it == bb_lde_fast_tab.end()) {
+    std::vector<uint32_t> po, pf;
+    host::bb_lde_fast_tables(log_n, added_bits, shift_std, lo, po, pf);
+    DevBuf a, b;
+    sfn_status s = upload_table(a, po.data(), po.size() * 4);
+    if (s) return s;
+    s = upload_table(b, pf.data(), pf.size() * 4);
+    if (s) return s;
+    it = bb_lde_fast_tab.emplace(key, std::make_pair(a, b)).first;
+  }
+  *post = (const uint32_t *)it->second.first.ptr;
+  *pointf = (const uint32_t *)it->second.second.ptr;
   return SFN_OK;
 }
 
@@ -284,17 +324,19 @@ void choose_lanes(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) 
 bool choose_lanes_fast(PassDesc &p, int layout, uint32_t batch, uint32_t want_log_w) {
   if (p.log_t < 6 || p.log_t > (int)kMaxLogTile) return false;
   uint32_t lw;
+  // lane orders whose addresses grow with the lane index, so lane 0 bounds the tile
   if (layout == SFN_LAYOUT_ROW_MAJOR) {
     p.order = kColFastest;
     p.lane_contig = 1;
     lw = want_log_w;
+    while (lw > 0 && (batch & ((1u << lw) - 1))) --lw;  // W divides the row
   } else if (p.lo > 0) {
     p.order = kLowFastest;
     p.lane_contig = 1;
     lw = std::min<uint32_t>(want_log_w, p.lo);
   } else {
     p.lane_contig = 0;
-    p.order = batch >= 4 ? kColFastest : kHighFastest;
+    p.order = kHighFastest;
     lw = p.log_t >= 13 ? 0 : 13 - p.log_t;  // 256 threads
   }
   while (lw > 0 && (((1u << lw) << (p.log_t - 5)) > 512 || fast_lds_bytes(1, p.log_t, lw) > kMaxLdsBytes)) --lw;
@@ -393,9 +435,24 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   return SFN_OK;
 }
 
+// Tuned kernels address a workgroup's tile with 32-bit buffer offsets from its
+// lane 0: the point extent plus the lane spread must stay below 4 GiB.
+static bool fast_offsets_fit(const PassDesc &p, Layout ls, Layout ld) {
+  const uint64_t T = 1ull << p.log_t, W = 1ull << p.log_w;
+  auto span = [&](Layout L) {
+    const uint64_t pts = (T << p.lo) * L.rs;
+    uint64_t lane;
+    if (p.order == kLowFastest) lane = W * L.rs;
+    else if (p.order == kColFastest) lane = W * std::max(L.cs, L.rs);
+    else lane = W * ((T << p.lo) * L.rs + L.cs);
+    return (pts + lane + 1) * 4;
+  };
+  return span(ls) < 0xFFFFFFFFull && span(ld) < 0xFFFFFFFFull;
+}
+
 static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src,
                               uint32_t *dst, Layout ls, Layout ld, uint32_t batch, uint32_t log_n_real,
-                              hipStream_t s) {
+                              hipStream_t s, uint64_t src_elems = 0, uint64_t dst_elems = 0) {
   if (p.layer_kernel) {
     LayerArgs a{};
     a.src = src;
@@ -442,11 +499,19 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   if (st0) return st0;
   // the point-adjacent tuned kernel moves 16-byte vectors: needs aligned buffers
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
-  if (p.fast && (p.lane_contig || aligned)) {
+  if (p.fast && (p.lane_contig || aligned) && src_elems && dst_elems && fast_offsets_fit(p, ls, ld)) {
+    a.src_elems = src_elems;
+    a.dst_elems = dst_elems;
     if (field == SFN_FIELD_BABYBEAR) {
       const uint2 *root;
       if ((st0 = ctx->bb_root_table(&root))) return st0;
       a.tw = root;
+      a.twist = nullptr;
+      if (p.lo > 0) {
+        const uint32_t *tm;
+        if ((st0 = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, &tm))) return st0;
+        a.twist_m = tm;
+      }
     }
     (void)ctx->timing_mark(s, true, std::string("k_fast<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
                                        std::to_string(p.log_t) + (p.lane_contig ? ",lanes" : ",points") +
@@ -459,7 +524,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   if (p.fast) {  // fall back to the generic kernel with its own geometry
     PassDesc q = p;
     q.fast = false;
-    choose_lanes(q, SFN_LAYOUT_SOA, batch, 4);
+    choose_lanes(q, ls.cs == 1 && ls.rs != 1 ? SFN_LAYOUT_ROW_MAJOR : SFN_LAYOUT_SOA, batch, 4);
     return launch_desc(ctx, field, dir, q, src, dst, ls, ld, batch, log_n_real, s);
   }
   (void)ctx->timing_mark(s, true, std::string("k_pass<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
@@ -500,7 +565,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
   for (size_t i = 0; i < passes.size(); ++i) {
     uint32_t *dst = d_out;
     if (pp) dst = ((passes.size() - 1 - i) % 2 == 0) ? d_out : pp;
-    sfn_status st = launch_desc(this, field, dir, passes[i], src, dst, L, L, batch, log_n, s);
+    sfn_status st = launch_desc(this, field, dir, passes[i], src, dst, L, L, batch, log_n, s, total, total);
     if (st) return st;
     src = dst;
   }
@@ -523,6 +588,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   const uint32_t lo = log_n - mid_t;
   const Layout Lin = layout_of(layout, log_n, batch);
   const Layout Lout = layout_of(layout, LN, batch);
+  const uint64_t n_in = (uint64_t)batch << log_n, n_out = (uint64_t)batch << LN;
   sfn_status st;
   auto outer = split_layers(lo, mlt);  // ascending (lo', log_t')
   // K1: inverse layers [0, lo) (no normalisation) into the first n rows of out
@@ -535,7 +601,8 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = log_n;
     if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
-    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, d_out, mid_ls, Lout, batch, log_n, s);
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, d_out, mid_ls, Lout, batch, log_n, s,
+                     mid_src == d_in ? n_in : n_out, n_out);
     if (st) return st;
     mid_src = d_out;
     mid_ls = Lout;
@@ -583,10 +650,20 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     q.lo = lo;
     q.log_t = (int)mid_t;
     q.log_n = log_n;
-    if (sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig) {
+    Layout mid_ld = Lout;  // per block the kernel rebases at the block start
+    if (sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig &&
+        fast_offsets_fit(q, mid_ls, mid_ld)) {
       const uint2 *root;
+      const uint32_t *tim, *post, *pointf;
       if ((st = bb_root_table(&root))) return st;
+      if ((st = bb_twist_m_table(lo, log_n, true, &tim))) return st;
+      if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, &post, &pointf))) return st;
       a.tw = root;
+      a.twist_m = tim;
+      a.post_m = post;
+      a.pointf_m = pointf;
+      a.src_elems = mid_src == d_in ? n_in : n_out;
+      a.dst_elems = n_out;
       a.log_w = q.log_w;
       a.order = q.order;
       a.lane_contig = 1;
@@ -612,7 +689,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = LN;
     if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
-    st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s);
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s, n_out, n_out);
     if (st) return st;
   }
   return SFN_OK;

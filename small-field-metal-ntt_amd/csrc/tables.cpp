@@ -69,6 +69,45 @@ std::vector<Pair> bb_lde_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t
   return t;
 }
 
+std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t) {
+  std::vector<uint32_t> m(t.size());
+  for (size_t i = 0; i < t.size(); ++i) m[i] = bb_to_monty(t[i].w);
+  return m;
+}
+
+void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
+                        std::vector<uint32_t> &post, std::vector<uint32_t> &pointf) {
+  const size_t n = (size_t)1 << log_n;
+  const uint32_t lt = log_n - lo;
+  const size_t T = (size_t)1 << lt;
+  const uint32_t blocks = 1u << added_bits;
+  const uint32_t inv_n = invmod((uint32_t)(n % kBBP), kBBP);
+  const uint32_t gN = bb_root(log_n + added_bits);
+  const std::vector<Pair> tw = bb_twist_table(lo, log_n, false);
+  post.assign(n * blocks, 0);
+  pointf.assign(T * blocks, 0);
+  for (uint32_t b = 0; b < blocks; ++b) {
+    const uint32_t sigma = mulmod(shift_std, powmod(gN, bitrev(b, added_bits), kBBP), kBBP);
+    std::vector<uint32_t> lane((size_t)1 << lo);
+    uint32_t w = inv_n;
+    for (size_t low = 0; low < lane.size(); ++low) {
+      lane[low] = w;  // n^-1 * sigma^low
+      w = mulmod(w, sigma, kBBP);
+    }
+    for (size_t p = 0; p < T; ++p)
+      for (size_t low = 0; low < lane.size(); ++low) {
+        const size_t i = (p << lo) | low;
+        post[b * n + i] = bb_to_monty(mulmod(tw[i].w, lane[low], kBBP));
+      }
+    const uint32_t step = powmod(sigma, (uint64_t)1 << lo, kBBP);  // sigma^(2^lo)
+    uint32_t f = 1;
+    for (size_t p = 0; p < T; ++p) {
+      pointf[b * T + p] = bb_to_monty(f);
+      f = mulmod(f, step, kBBP);
+    }
+  }
+}
+
 namespace {
 struct Pt {
   uint32_t x, y;

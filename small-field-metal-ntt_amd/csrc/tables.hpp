@@ -63,6 +63,16 @@ std::vector<Pair> bb_root_table(uint32_t log);
 // g_hi^(+-low * bitrev(p, hi - lo)).
 std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse);
 
+// Single-word Montgomery form (w * 2^32 mod p) of a table of standard values.
+std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t);
+
+// Coset-LDE tuned middle pass, sigma_b = shift * g_{L+a}^bitrev(b, a):
+//  post:   block b entry (p << lo) | low = g_L^(low*bitrev(p, L-lo)) * n^-1 * sigma_b^low
+//  pointf: block b entry p (p < 2^(L-lo)) = sigma_b^(p << lo)
+// both in Montgomery form.
+void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
+                        std::vector<uint32_t> &post, std::vector<uint32_t> &pointf);
+
 // Coset-LDE middle factors: block b (of 2^a) entry j holds
 // n^-1 * (shift * g_{L+a}^bitrev(b, a))^j, j < 2^L.
 std::vector<Pair> bb_lde_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std);
