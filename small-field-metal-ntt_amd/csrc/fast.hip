@@ -76,6 +76,8 @@ __device__ __forceinline__ uint32_t lidx(uint32_t lane, uint32_t p, uint32_t log
 // workgroup's first lane (lane 0 has the lowest address of the tile).
 struct RoundIO {
   Rsrc src, dst, twist;
+  const uint32_t *srcp;  // plain pointers (point-adjacent variant), already offset to lane 0
+  uint32_t *dstp;
   uint64_t sbase0, dbase0;  // element index of lane 0's row 0 in src / dst
   uint32_t sps, dps;        // elements between consecutive points
   uint32_t lo;
@@ -105,12 +107,20 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     if constexpr (FIRST) {
       if (li.valid) {
         const uint32_t voff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pbase * io.sps) * 4u;
-        if constexpr (!CONTIG && B0 == 0 && K >= 2) {
-          // R consecutive words per thread (points are adjacent rows)
+        if constexpr (!CONTIG) {
+          // points are adjacent rows: plain global loads, 16-byte vectors when
+          // a thread owns R consecutive points
+          const uint32_t *sp = io.srcp + (li.sbase - io.sbase0) + (uint64_t)pbase * io.sps;
+          if constexpr (B0 == 0 && K >= 2) {
+            const uint4 *s4 = reinterpret_cast<const uint4 *>(sp);
 #pragma unroll
-          for (int i = 0; i < R / 4; ++i) {
-            const uint4 q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(io.src, (int)(voff + 16 * i), 0, 0));
-            v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+            for (int i = 0; i < R / 4; ++i) {
+              const uint4 q = s4[i];
+              v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[i] = sp[(uint64_t)(i << B0) * io.sps];
           }
         } else {
           const uint32_t step = (io.sps << B0) * 4u;
@@ -159,13 +169,16 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
         }
       }
       const uint32_t voff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pbase * io.dps) * 4u;
-      if constexpr (!CONTIG && B0 == 0 && K >= 2) {
+      if constexpr (!CONTIG) {
+        uint32_t *dp = io.dstp + (li.dbase - io.dbase0) + (uint64_t)pbase * io.dps;
+        if constexpr (B0 == 0 && K >= 2) {
+          uint4 *d4 = reinterpret_cast<uint4 *>(dp);
 #pragma unroll
-        for (int i = 0; i < R / 4; ++i)
-          __builtin_amdgcn_raw_buffer_store_b128(
-              __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int,
-                                 make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3])),
-              io.dst, (int)(voff + 16 * i), 0, 0);
+          for (int i = 0; i < R / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < R; ++i) dp[(uint64_t)(i << B0) * io.dps] = v[i];
+        }
       } else {
         const uint32_t step = (io.dps << B0) * 4u;
 #pragma unroll
@@ -232,6 +245,8 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.dbase0 = l0.dbase;
   io.src = make_rsrc(a.src + l0.sbase, (a.src_elems - l0.sbase) * 4);
   io.dst = make_rsrc(a.dst + l0.dbase, (a.dst_elems - l0.dbase) * 4);
+  io.srcp = a.src + l0.sbase;
+  io.dstp = a.dst + l0.dbase;
   io.sps = (uint32_t)(((uint64_t)1 << a.lo) * a.src_rs);
   io.dps = (uint32_t)(((uint64_t)1 << a.lo) * a.dst_rs);
   io.lo = a.lo;

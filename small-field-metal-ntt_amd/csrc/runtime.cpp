@@ -2,6 +2,7 @@
 // executor, and the host-pointer entry points.
 #include "runtime.hpp"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -499,7 +500,8 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   if (st0) return st0;
   // the point-adjacent tuned kernel moves 16-byte vectors: needs aligned buffers
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
-  if (p.fast && (p.lane_contig || aligned) && src_elems && dst_elems && fast_offsets_fit(p, ls, ld)) {
+  if (p.fast && ctx->allow_fast && (p.lane_contig || aligned) && src_elems && dst_elems &&
+      fast_offsets_fit(p, ls, ld)) {
     a.src_elems = src_elems;
     a.dst_elems = dst_elems;
     if (field == SFN_FIELD_BABYBEAR) {
@@ -651,7 +653,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     q.log_t = (int)mid_t;
     q.log_n = log_n;
     Layout mid_ld = Lout;  // per block the kernel rebases at the block start
-    if (sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig &&
+    if (allow_fast && sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig &&
         fast_offsets_fit(q, mid_ls, mid_ld)) {
       const uint2 *root;
       const uint32_t *tim, *post, *pointf;
@@ -774,6 +776,8 @@ sfn_status sfn_ctx_create(int device, sfn_ctx **out) {
     delete c;
     return SFN_ERR_GPU_EXECUTION;
   }
+  const char *generic_only = getenv("SFNTT_GENERIC_ONLY");
+  x.allow_fast = !(generic_only && generic_only[0] == '1');
   if (strncmp(x.props.gcnArchName, "gfx950", 6) != 0) {
     delete c;
     return SFN_ERR_DEVICE_NOT_FOUND;  // the kernels are built for gfx950 only

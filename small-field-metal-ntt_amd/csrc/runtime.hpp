@@ -77,6 +77,7 @@ class Context {
   DevBuf scratch_in, scratch_out, scratch_pp;
 
   std::map<TuneKey, Schedule> tuned;
+  bool allow_fast = true;  // SFNTT_GENERIC_ONLY=1 at context creation turns the tuned kernels off
 
   // per-kernel event timing of the last launch sequence on `stream`
   bool pass_timing = false;
