@@ -31,7 +31,7 @@ COUNTERS_FILE = os.path.join(ROOT, "profiles", "lde_2e20x256_hbm_counters.txt")
 def kernel_bytes(label: str, log_n: int, batch: int, added_bits: int) -> int:
     """Algorithmic bytes of one launch: each value it must read once + each it writes once."""
     n, N = batch << log_n, batch << (log_n + added_bits)
-    if label.startswith("k_lde_mid"):
+    if "mid" in label.split("<")[0]:  # fused middle pass: n in, N out
         return 4 * (n + N)
     if "inv" in label:  # iDFT low layers: n in, n out
         return 8 * n
@@ -72,6 +72,26 @@ def cpu_baseline(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw
     return res
 
 
+def cpu_baseline_ntt(workload: str) -> dict:
+    """The oracle's restatement of the reference CPU forward NTT (1 thread), 8
+    columns in one call so the twiddle table is built once (the reference's
+    TwiddleCache keeps it between calls)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    from sfntt import synth
+
+    field, log_n = (1, 16) if workload == "bb16" else (0, 20)
+    n, cols = 1 << log_n, 8
+    x = np.concatenate([synth.field_data(field, 12345 + n, n)] * cols)
+    orc.lib()
+    t0 = time.perf_counter()
+    orc.ntt_batch(field, False, log_n, cols, x, threads=1)
+    dt = (time.perf_counter() - t0) / cols
+    return {"value": round(n / dt / 1e6, 3), "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": f"{cols} columns of 2^{log_n} in one call, {dt * 1e6:.0f} us per column, 1 thread"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +102,9 @@ def main() -> int:
     ap.add_argument("--added-bits", type=int, default=1)
     ap.add_argument("--semantics", choices=["as-stated", "reference"], default="as-stated")
     ap.add_argument("--cpu-cols", type=int, default=24, help="columns in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "uneven"], default="lde")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -102,6 +125,23 @@ def main() -> int:
     ctx = sfntt.HipContext(0)
     lib = ctx._lib
     info = ctx.device_info()
+    if args.copy_probe:
+        # identity transform (log_n = 0) out of place = one plain copy kernel:
+        # reads 1 GiB and writes 1 GiB
+        cin, cout = sfntt.DeviceBuffer(ctx, 1 << 28), sfntt.DeviceBuffer(ctx, 1 << 28)
+        ctx.check(lib.sfn_ntt_device(ctx.ptr, 1, sfntt.VARIANT_AUTO, 0, 0, 1 << 28, sfntt.LAYOUT_SOA,
+                                     cin.ptr, cout.ptr, None))
+        ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+        cin.free()
+        cout.free()
+    if args.workload != "lde":
+        import workloads  # small-field-metal-ntt_amd/workloads.py
+        res = workloads.run(args.workload, ctx, args.steps, args.warmup)
+        res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "vs_baseline": None})
+        if args.cpu_cols > 0 and args.workload in ("bb16", "m31_20"):
+            res["cpu_baseline"] = cpu_baseline_ntt(args.workload)
+        print(json.dumps(res), flush=True)
+        return 0
     # reference generation order: column-major stream, seed 42 + log_n (benches/ntt_benchmark.rs:482-490)
     x = synth.field_data(1, 42 + log_n, n * B)
     x_dev = x.reshape(B, n).T.copy().ravel() if row_major else x

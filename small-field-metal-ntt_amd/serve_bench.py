@@ -1,0 +1,168 @@
+"""Serving benchmark: open-loop Poisson arrivals of mixed NTT / coset-LDE jobs.
+
+Job mix (seeded): 60% BabyBear coset LDE (2^14..2^20 rows x 8..256 columns, 2x or
+4x blowup), 40% M31 circle NTT (2^12..2^22, one column).  Columns stay resident
+in HBM: inputs are device buffers, every job writes its own device output.
+
+Two systems on the same GPU and the same arrival trace:
+  plain  - one job per launch sequence, in arrival order, synchronous (the
+           reference's calling pattern: one call, wait, next call);
+  server - the continuous-batching loop (sfn_server_*): FIFO admission by HBM
+           footprint, several worker streams, same-shape jobs fused.
+For each offered rate the p50/p99 latency (arrival -> result complete) and
+the completed jobs/s are reported; the headline is the highest rate each
+system sustains under a p99 target, and their ratio.
+
+    python small-field-metal-ntt_amd/serve_bench.py [--jobs 200] [--rates 50,100,...] [--p99-ms 100]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import sfntt  # noqa: E402
+from sfntt import _lib as L  # noqa: E402
+from sfntt import synth  # noqa: E402
+
+
+def draw_jobs(seed: int, count: int, scale: int = 0):
+    """(kind, log_n, batch, added_bits) per job; `scale` shrinks sizes for quick runs."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(count):
+        if rng.random() < 0.6:
+            jobs.append(("lde", int(rng.integers(14, 21)) - scale, int(rng.integers(8, 257)),
+                         int(rng.integers(1, 3))))
+        else:
+            jobs.append(("ntt", int(rng.integers(12, 23)) - scale, 1, 0))
+    return jobs
+
+
+class Pool:
+    """Device buffers: one shared read-only input per size, one output per job."""
+
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        max_in = max(b << n for _, n, b, _ in jobs)
+        x = synth.field_data(1, 0x5EED0005, max_in)
+        self.inp = sfntt.DeviceBuffer(ctx, max_in)
+        self.inp.upload(x)
+        # M31 inputs must be < 2^31 - 1: BabyBear values are < p_BB < p_M31, valid for both
+        self.outs = [sfntt.DeviceBuffer(ctx, b << (n + a)) for _, n, b, a in jobs]
+
+
+def run_plain(ctx, jobs, pool, arrivals):
+    lib = ctx._lib
+    lat = []
+    t0 = time.perf_counter()
+    for k, ((kind, n, b, a), at) in enumerate(zip(jobs, arrivals)):
+        now = time.perf_counter() - t0
+        if now < at:
+            time.sleep(at - now)
+        if kind == "lde":
+            st = lib.sfn_coset_lde_batch_device(ctx.ptr, n, b, a, 0, L.LAYOUT_SOA, pool.inp.ptr,
+                                                pool.outs[k].ptr, None)
+        else:
+            st = lib.sfn_ntt_device(ctx.ptr, L.FIELD_M31, L.VARIANT_AUTO, L.FORWARD, n, 1, L.LAYOUT_SOA,
+                                    pool.inp.ptr, pool.outs[k].ptr, None)
+        ctx.check(st)
+        ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+        lat.append(time.perf_counter() - t0 - at)
+    span = time.perf_counter() - t0
+    return np.array(lat), span
+
+
+def run_server(ctx, jobs, pool, arrivals, inflight=4, group=64):
+    lib = ctx._lib
+    cfg = L.ServerConfig(0, inflight, 0, group, 0)
+    srv = ctypes.c_void_p()
+    ctx.check(lib.sfn_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(srv)))
+    tickets = []
+    base = lib.sfn_now()
+    t0 = time.perf_counter()
+    for k, ((kind, n, b, a), at) in enumerate(zip(jobs, arrivals)):
+        now = time.perf_counter() - t0
+        if now < at:
+            time.sleep(at - now)
+        if kind == "lde":
+            r = L.Request(L.JOB_LDE, L.FIELD_BABYBEAR, 0, n, b, a, 0, 1, pool.inp.ptr, pool.outs[k].ptr)
+        else:
+            r = L.Request(L.JOB_NTT, L.FIELD_M31, L.FORWARD, n, 1, 0, 0, 1, pool.inp.ptr, pool.outs[k].ptr)
+        t = ctypes.c_uint64()
+        ctx.check(lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+        tickets.append((t.value, at))
+    lat, groups = [], []
+    end_max = 0
+    for t, at in tickets:
+        tm = L.JobTiming()
+        ctx.check(lib.sfn_server_wait(srv, t, ctypes.byref(tm)))
+        # latency from the scheduled arrival (the submit happens at it)
+        lat.append((tm.end_ns - base) / 1e9 - at)
+        groups.append(tm.group_size)
+        end_max = max(end_max, tm.end_ns)
+    lib.sfn_server_destroy(srv)
+    return np.array(lat), (end_max - base) / 1e9, float(np.mean(groups))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--rates", default="")
+    ap.add_argument("--p99-ms", type=float, default=100.0)
+    ap.add_argument("--scale", type=int, default=0, help="shrink every log_n by this much (quick runs)")
+    ap.add_argument("--inflight", type=int, default=4)
+    args = ap.parse_args()
+    ctx = sfntt.HipContext(0)
+    jobs = draw_jobs(0x5EED0005, args.jobs, args.scale)
+    pool = Pool(ctx, jobs)
+    # warm every shape's tables once, outside the timed traces
+    for k, (kind, n, b, a) in enumerate(jobs):
+        if kind == "lde":
+            ctx.check(ctx._lib.sfn_coset_lde_batch_device(ctx.ptr, n, b, a, 0, 0, pool.inp.ptr, pool.outs[k].ptr, None))
+        else:
+            ctx.check(ctx._lib.sfn_ntt_device(ctx.ptr, 0, 4, 0, n, 1, 0, pool.inp.ptr, pool.outs[k].ptr, None))
+    ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+    # service-time estimate: plain path at saturation (all arrivals at t=0)
+    lat0, span0 = run_plain(ctx, jobs, pool, np.zeros(len(jobs)))
+    cap_plain = len(jobs) / span0
+    rates = [float(r) for r in args.rates.split(",")] if args.rates else \
+        [round(cap_plain * f, 2) for f in (0.3, 0.5, 0.7, 0.85, 1.0, 1.2, 1.5, 2.0, 3.0)]
+    rng = np.random.default_rng(0x5EED0005 + 1)
+    gaps = rng.exponential(1.0, len(jobs))
+    rows = []
+    for lam in rates:
+        arr = np.cumsum(gaps / lam) - gaps[0] / lam
+        lp, sp = run_plain(ctx, jobs, pool, arr)
+        ls, ss, g = run_server(ctx, jobs, pool, arr, args.inflight)
+        row = {"offered_jobs_s": lam,
+               "plain": {"jobs_s": round(len(jobs) / sp, 2), "p50_ms": round(np.percentile(lp, 50) * 1e3, 3),
+                         "p99_ms": round(np.percentile(lp, 99) * 1e3, 3)},
+               "server": {"jobs_s": round(len(jobs) / ss, 2), "p50_ms": round(np.percentile(ls, 50) * 1e3, 3),
+                          "p99_ms": round(np.percentile(ls, 99) * 1e3, 3), "mean_group": round(g, 2)}}
+        print(json.dumps(row), flush=True)
+        rows.append(row)
+    target = args.p99_ms
+
+    def best(sys_):
+        ok = [r for r in rows if r[sys_]["p99_ms"] <= target]
+        return max((r[sys_]["jobs_s"] for r in ok), default=0.0)
+
+    bp, bs = best("plain"), best("server")
+    print(json.dumps({"metric": "serving jobs/s at p99 <= target (device-resident columns)",
+                      "p99_target_ms": target, "plain_jobs_s": bp, "server_jobs_s": bs,
+                      "ratio": round(bs / bp, 3) if bp else None, "jobs_per_trace": len(jobs),
+                      "plain_capacity_jobs_s": round(cap_plain, 2)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -51,3 +51,24 @@ def lcg_u32(seed: int, n: int, modulus: int, monty: bool, out: np.ndarray | None
 def field_data(field: int, seed: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
     """field 0 = M31 (canonical), 1 = BabyBear (Montgomery raw)."""
     return lcg_u32(seed, n, M31_P if field == 0 else BB_P, field == 1, out)
+
+
+def splitmix64(state: int):
+    """SplitMix64 (Steele, Lea, Flood 2014): returns (next_state, output)."""
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def uneven_columns(seed: int = 0x5EED0004, count: int = 512, lo: int = 10, hi: int = 22):
+    """Uneven batch draw: field ~ Bernoulli(1/2), log_n ~ U{lo..hi}, per-column LCG seed."""
+    st = seed
+    out = []
+    for _ in range(count):
+        st, a = splitmix64(st)
+        st, b = splitmix64(st)
+        st, c = splitmix64(st)
+        out.append((a & 1, lo + b % (hi - lo + 1), c))
+    return out

@@ -1,0 +1,122 @@
+"""Secondary benchmark workloads (bench.py --workload ...), one JSON dict each.
+
+bb16    BabyBear forward NTT, one column of 2^16, every variant
+        (benches/ntt_benchmark.rs:632-700 shootout shape, seed 12345 + n)
+m31_20  M31 circle forward NTT, one column of 2^20, every variant, twiddles
+        resident on the device (benches/ntt_benchmark.rs:183-375 shape)
+uneven  512 BabyBear/M31 columns with log_n ~ U{10..22} in one multi-job plan
+        (one launch per pass, hipGraph replay) against launching every column
+        on its own
+
+Device time is a hipEvent pair around the launch sequence with inputs
+resident in HBM; "call" time is the host-pointer entry point's wall clock
+(upload + launches + download), the boundary the reference's shootout timed.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+import sfntt
+from sfntt import _lib as L
+from sfntt import synth
+
+VARIANTS = [("v1_ct_dit_r2", L.CT_DIT_R2), ("v2_ct_gs_r2", L.CT_GS_R2), ("v3_stockham_r2", L.STOCKHAM_R2),
+            ("v4_ct_gs_r4", L.CT_GS_R4), ("auto", L.VARIANT_AUTO)]
+
+
+def _single(ctx, field: int, log_n: int, steps: int, warmup: int) -> dict:
+    lib = ctx._lib
+    n = 1 << log_n
+    x = synth.field_data(field, 12345 + n, n)
+    din, dout = sfntt.DeviceBuffer(ctx, n), sfntt.DeviceBuffer(ctx, n)
+    din.upload(x)
+    ctx.twiddles_prepare(field, log_n)
+    res = {}
+    ns = ctypes.c_uint64()
+    for name, v in VARIANTS:
+        dev = []
+        for k in range(warmup + steps):
+            ctx.check(lib.sfn_ntt_device(ctx.ptr, field, v, L.FORWARD, log_n, 1, L.LAYOUT_SOA, din.ptr, dout.ptr,
+                                         None))
+            ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+            if k >= warmup:
+                dev.append(ns.value / 1e3)
+        wall = []
+        for k in range(warmup + steps):
+            t0 = time.perf_counter()
+            ctx.ntt(field, v, L.FORWARD, log_n, 1, x)
+            if k >= warmup:
+                wall.append((time.perf_counter() - t0) * 1e6)
+        d, w = float(np.median(dev)), float(np.median(wall))
+        res[name] = {"device_us": round(d, 2), "call_us": round(w, 1),
+                     "device_melem_s": round(n / d, 1), "call_melem_s": round(n / w, 1)}
+    return res
+
+
+def run(name: str, ctx, steps: int, warmup: int) -> dict:
+    if name in ("bb16", "m31_20"):
+        field, log_n = (L.FIELD_BABYBEAR, 16) if name == "bb16" else (L.FIELD_M31, 20)
+        r = _single(ctx, field, log_n, max(steps, 20), max(warmup, 5))
+        best = min(r.values(), key=lambda e: e["device_us"])
+        return {"metric": f"{'BabyBear' if field else 'M31'} forward NTT 2^{log_n} (one column), device Melem/s",
+                "value": best["device_melem_s"], "unit": "Melem/s", "higher_is_better": True,
+                "dtype": "u32", "data": "synthetic (reference LCG, seed 12345+n)",
+                "config": {"workload": f"{'bb' if field else 'm31'}_forward_2^{log_n}_single_column",
+                           "log_n": log_n, "batch": 1}, "variants": r}
+    if name == "uneven":
+        return _uneven(ctx, steps, warmup)
+    raise ValueError(name)
+
+
+def _uneven(ctx, steps: int, warmup: int) -> dict:
+    lib = ctx._lib
+    cols = synth.uneven_columns()
+    bufs, descs, total = [], [], 0
+    for f, log_n, seed in cols:
+        x = synth.field_data(f, seed, 1 << log_n)
+        b = sfntt.DeviceBuffer(ctx, x.size)
+        b.upload(x)
+        bufs.append(b)
+        descs.append(L.JobDesc(f, L.FORWARD, log_n, 1, b.ptr, b.ptr))
+        total += 1 << log_n
+    arr = (L.JobDesc * len(descs))(*descs)
+    out = {}
+    for use_graph in (1, 0):
+        plan = ctypes.c_void_p()
+        ctx.check(lib.sfn_plan_create(ctx.ptr, arr, len(descs), use_graph, ctypes.byref(plan)))
+        nl, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        ctx.check(lib.sfn_plan_info(plan, ctypes.byref(nl), ctypes.byref(nt)))
+        for _ in range(warmup):
+            ctx.check(lib.sfn_plan_launch(plan, None))
+        ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.check(lib.sfn_plan_launch(plan, None))
+        ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+        dt = (time.perf_counter() - t0) / steps
+        out["plan_graph" if use_graph else "plan_direct"] = {
+            "ms": round(dt * 1e3, 3), "melem_s": round(total / dt / 1e6, 1), "launches": nl.value,
+            "tiles": nt.value}
+        lib.sfn_plan_destroy(plan)
+    # one launch sequence per column (tuned kernels, same stream, no waits in between)
+    for _ in range(warmup):
+        for (f, log_n, _), b in zip(cols, bufs):
+            ctx.check(lib.sfn_ntt_device(ctx.ptr, f, L.VARIANT_AUTO, L.FORWARD, log_n, 1, L.LAYOUT_SOA, b.ptr, b.ptr,
+                                         None))
+    ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for (f, log_n, _), b in zip(cols, bufs):
+            ctx.check(lib.sfn_ntt_device(ctx.ptr, f, L.VARIANT_AUTO, L.FORWARD, log_n, 1, L.LAYOUT_SOA, b.ptr, b.ptr,
+                                         None))
+    ctx.check(lib.sfn_synchronize(ctx.ptr, None))
+    dt = (time.perf_counter() - t0) / steps
+    out["per_column"] = {"ms": round(dt * 1e3, 3), "melem_s": round(total / dt / 1e6, 1), "launches": None}
+    return {"metric": "uneven batch: 512 BabyBear+M31 columns, log_n ~ U{10..22}, forward, Melem/s",
+            "value": out["plan_graph"]["melem_s"], "unit": "Melem/s", "higher_is_better": True, "dtype": "u32",
+            "data": "synthetic (SplitMix64 seed 0x5EED0004 shapes, reference LCG values)",
+            "config": {"workload": "uneven_512_columns_log10to22", "columns": len(cols), "elements": total},
+            "modes": out}
