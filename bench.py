@@ -72,6 +72,51 @@ def cpu_baseline(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw
     return res
 
 
+def cpu_baseline_all_cores(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw: int) -> dict:
+    """Same restatement with the columns split over every host thread (at most 16)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    orc.lib()
+    t0 = time.perf_counter()
+    orc.coset_lde_batch(x_soa_sample, log_n, cols, added_bits, shift_raw, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cols * (1 << (log_n + added_bits)) / dt / 1e6, 3), "unit": "Melem/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{cols} of 256 columns, {dt:.2f} s, columns split over {threads} threads"}
+
+
+def end_to_end(ctx, x_dev, log_n: int, B: int, a: int, shift_raw: int, layout: int, steps: int) -> dict:
+    """Host-pointer entry point (upload -> LDE -> download) on pinned host buffers."""
+    import numpy as np
+
+    lib = ctx._lib
+    n_in, n_out = x_dev.size, B << (log_n + a)
+    hin, hout = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx.check(lib.sfn_host_alloc(ctx.ptr, n_in * 4, ctypes.byref(hin)))
+    ctx.check(lib.sfn_host_alloc(ctx.ptr, n_out * 4, ctypes.byref(hout)))
+    try:
+        ctypes.memmove(hin, x_dev.ctypes.data, n_in * 4)
+        dev_ns = ctypes.c_uint64()
+        pin = ctypes.cast(hin, ctypes.POINTER(ctypes.c_uint32))
+        pout = ctypes.cast(hout, ctypes.POINTER(ctypes.c_uint32))
+        times = []
+        for k in range(steps + 1):
+            t0 = time.perf_counter()
+            ctx.check(lib.sfn_coset_lde_batch(ctx.ptr, log_n, B, a, shift_raw, layout, pin, pout,
+                                              ctypes.byref(dev_ns)))
+            if k:
+                times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
+        return {"ms": round(dt * 1e3, 3), "melem_s": round(n_out / dt / 1e6, 2),
+                "pcie_bytes": 4 * (n_in + n_out), "host_buffers": "pinned (sfn_host_alloc)",
+                "steps": steps}
+    finally:
+        lib.sfn_host_free(ctx.ptr, hin)
+        lib.sfn_host_free(ctx.ptr, hout)
+
+
 def cpu_baseline_ntt(workload: str) -> dict:
     """The oracle's restatement of the reference CPU forward NTT (1 thread), 8
     columns in one call so the twiddle table is built once (the reference's
@@ -105,6 +150,10 @@ def main() -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "uneven"], default="lde")
+    ap.add_argument("--e2e-steps", type=int, default=3,
+                    help="timed host-pointer (PCIe-inclusive) LDE calls on pinned buffers (0 = skip)")
+    ap.add_argument("--cpu-all-cols", type=int, default=128,
+                    help="columns in the all-cores CPU baseline sample (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -195,6 +244,15 @@ def main() -> int:
         y = d_out.download()
         out_first = (y.reshape(N, B).T[:cols].ravel() if row_major else y[:cols * N])
         cb = cpu_baseline(x[:cols * n], log_n, cols, a, shift_raw, out_first)
+    cb_all = None
+    if args.cpu_all_cols > 0:
+        cols = min(args.cpu_all_cols, B)
+        cb_all = cpu_baseline_all_cores(x[:cols * n], log_n, cols, a, shift_raw)
+    e2e = None
+    if args.e2e_steps > 0:
+        d_in.free()
+        d_out.free()
+        e2e = end_to_end(ctx, x_dev, log_n, B, a, shift_raw, layout, args.e2e_steps)
 
     line = {
         "metric": "coset-LDE Melem/s (output elements) + % HBM roofline",
@@ -230,6 +288,8 @@ def main() -> int:
             "kernels_us": {k: round(v, 1) for k, v in kernels.items()},
         },
         "cpu_baseline": cb,
+        "cpu_baseline_all_cores": cb_all,
+        "end_to_end": e2e,
     }
     print(json.dumps(line), flush=True)
     return 0

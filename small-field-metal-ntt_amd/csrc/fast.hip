@@ -237,7 +237,7 @@ __device__ __forceinline__ void load_root(const PassArgs &a, uint2 *tab) {
 template <int LOG_T>
 __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundIO &io) {
   const uint32_t log_w = a.log_w;
-  const uint64_t g0 = (uint64_t)blockIdx.x << log_w;
+  const uint64_t g0 = (uint64_t)xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log) << log_w;
   for (uint32_t l = threadIdx.x; l < (1u << log_w); l += blockDim.x) decode_lane(a, LOG_T, g0 + l, lanes[l]);
   LaneInfo l0;
   decode_lane(a, LOG_T, g0, l0);

@@ -46,6 +46,7 @@ struct PassArgs {
   const uint32_t *twist_m;        // four-step twist of this pass, Montgomery form
   const uint32_t *post_m;         // LDE: per-block post-twist x n^-1 * sigma_b^low (blocks x 2^log_n)
   const uint32_t *pointf_m;       // LDE: per-block point factors sigma_b^(p << lo) (blocks x 2^LOG_T)
+  uint32_t xcd_share_log;         // 2^s consecutive tiles share 128-byte lines: keep them on one XCD
 };
 
 // V1 (one launch per layer) arguments.

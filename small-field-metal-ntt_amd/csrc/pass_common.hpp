@@ -143,6 +143,18 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
   (void)R;
 }
 
+// Tile of workgroup g.  Workgroup ids go to the 8 XCDs round-robin (id mod 8)
+// and every XCD has its own L2, so 2^s consecutive tiles that share 128-byte
+// lines are given ids that land on one XCD: in each group of 8 * 2^s ids,
+// id r -> tile (r mod 8) * 2^s + r / 8.  Ids past the last full group keep
+// their tile.  s = 0 is the identity.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t g, uint32_t grid, uint32_t s) {
+  const uint32_t gs = 8u << s;
+  if (s == 0 || g >= grid - grid % gs) return g;
+  const uint32_t r = g & (gs - 1);
+  return (g - r) + ((r & 7u) << s) + (r >> 3);
+}
+
 template <int K, int B0>
 __device__ __forceinline__ uint32_t round_pbase(uint32_t rb) {
   return (rb & ((1u << B0) - 1)) | ((rb >> B0) << (B0 + K));

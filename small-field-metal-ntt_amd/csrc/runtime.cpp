@@ -349,6 +349,13 @@ bool choose_lanes_fast(PassDesc &p, int layout, uint32_t batch, uint32_t want_lo
   return true;
 }
 
+// Tuned passes whose lanes are adjacent elements: a tile row spans 4 * W
+// bytes, so 128 / (4 * W) consecutive tiles share every 128-byte line.
+uint32_t xcd_share_log(const Context *ctx, const PassDesc &p) {
+  if (!ctx->xcd_remap || !p.lane_contig || p.log_w >= 5) return 0;
+  return std::min<uint32_t>(3, 5 - p.log_w);
+}
+
 }  // namespace
 
 std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
@@ -504,6 +511,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
       fast_offsets_fit(p, ls, ld)) {
     a.src_elems = src_elems;
     a.dst_elems = dst_elems;
+    a.xcd_share_log = xcd_share_log(ctx, p);
     if (field == SFN_FIELD_BABYBEAR) {
       const uint2 *root;
       if ((st0 = ctx->bb_root_table(&root))) return st0;
@@ -669,6 +677,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       a.log_w = q.log_w;
       a.order = q.order;
       a.lane_contig = 1;
+      a.xcd_share_log = xcd_share_log(this, q);
       (void)timing_mark(s, true, "k_mid_fast<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
       hipError_t e = launch_lde_mid_fast((int)mid_t, a, q.tiles, s);
       if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
@@ -778,6 +787,8 @@ sfn_status sfn_ctx_create(int device, sfn_ctx **out) {
   }
   const char *generic_only = getenv("SFNTT_GENERIC_ONLY");
   x.allow_fast = !(generic_only && generic_only[0] == '1');
+  const char *xcd = getenv("SFNTT_XCD_REMAP");
+  x.xcd_remap = !(xcd && xcd[0] == '0');
   if (strncmp(x.props.gcnArchName, "gfx950", 6) != 0) {
     delete c;
     return SFN_ERR_DEVICE_NOT_FOUND;  // the kernels are built for gfx950 only

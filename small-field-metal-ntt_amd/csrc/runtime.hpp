@@ -78,6 +78,7 @@ class Context {
 
   std::map<TuneKey, Schedule> tuned;
   bool allow_fast = true;  // SFNTT_GENERIC_ONLY=1 at context creation turns the tuned kernels off
+  bool xcd_remap = true;   // SFNTT_XCD_REMAP=0 at context creation keeps workgroup id = tile
 
   // per-kernel event timing of the last launch sequence on `stream`
   bool pass_timing = false;
