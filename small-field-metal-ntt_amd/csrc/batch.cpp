@@ -165,15 +165,17 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
     if ((log_n + mlt - 1) / mlt > 3) continue;
     if (lde && (log_n > 2 * mlt)) continue;
     for (uint32_t logr : {3u, 4u, 5u})
-      for (uint32_t lw : {2u, 3u, 4u, 5u}) {
-        Schedule s;
-        s.max_log_t = mlt;
-        s.logr = logr;
-        s.log_w = lw;
-        s.mid_log_w = lw;
-        if (mlt + lw > 15) continue;
-        cands.push_back(s);
-      }
+      for (uint32_t lw : {2u, 3u, 4u, 5u})
+        for (uint32_t mlw : {2u, 3u, 4u, 5u}) {
+          if (!lde && mlw != lw) continue;  // the middle-pass width only matters to the LDE
+          Schedule s;
+          s.max_log_t = mlt;
+          s.logr = logr;
+          s.log_w = lw;
+          s.mid_log_w = mlw;
+          if (mlt + std::max(lw, mlw) > 15) continue;
+          cands.push_back(s);
+        }
   }
   float best = 1e30f;
   Schedule bs = c.schedule_for(lde ? 2 : field, log_n, batch, layout);

@@ -81,6 +81,7 @@ struct RoundIO {
   uint64_t sbase0, dbase0;  // element index of lane 0's row 0 in src / dst
   uint32_t sps, dps;        // elements between consecutive points
   uint32_t lo;
+  uint32_t tps, tls;  // twist / post entry of (p, low) = (p << tps) + (low << tls)
   bool has_twist;
   uint32_t has_scale, scale_w, scale_q;
   // LDE blocks: per-point factor table (one block) and post-twist table (one block)
@@ -129,8 +130,8 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
         }
         if constexpr (F::ID == 1 && DIR == 1) {
           if (io.has_twist) {
-            const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
-            const uint32_t tstep = (4u << (B0 + io.lo));
+            const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
+            const uint32_t tstep = (4u << (B0 + io.tps));
 #pragma unroll
             for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
           }
@@ -149,14 +150,14 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
       if (!li.valid) continue;
       if constexpr (F::ID == 1 && DIR == 0) {
         if (io.has_twist) {
-          const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
-          const uint32_t tstep = (4u << (B0 + io.lo));
+          const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
+          const uint32_t tstep = (4u << (B0 + io.tps));
 #pragma unroll
           for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
         }
         if (io.has_post) {
-          const uint32_t toff = (((uint32_t)pbase << io.lo) | li.low) * 4u;
-          const uint32_t tstep = (4u << (B0 + io.lo));
+          const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
+          const uint32_t tstep = (4u << (B0 + io.tps));
 #pragma unroll
           for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.post, toff, i * tstep));
         }
@@ -250,6 +251,8 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.sps = (uint32_t)(((uint64_t)1 << a.lo) * a.src_rs);
   io.dps = (uint32_t)(((uint64_t)1 << a.lo) * a.dst_rs);
   io.lo = a.lo;
+  io.tps = a.tab_low_major ? 0u : a.lo;
+  io.tls = a.tab_low_major ? (uint32_t)LOG_T : 0u;
   io.has_twist = a.twist_m != nullptr;
   if (io.has_twist) io.twist = make_rsrc(a.twist_m, (uint64_t)4 << (a.lo + LOG_T));
   io.has_scale = a.has_scale;

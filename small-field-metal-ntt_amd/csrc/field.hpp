@@ -109,4 +109,12 @@ __device__ __forceinline__ void bfly_gs<BabyBear>(uint32_t &x, uint32_t &y, Baby
   y = BabyBear::mul_tw(a + BabyBear::P - b, w);
 }
 
+// Either butterfly with twiddle 1: (x, y) -> (x + y, x - y).
+template <class F>
+__device__ __forceinline__ void bfly_unit(uint32_t &x, uint32_t &y) {
+  const uint32_t a = x, b = y;
+  x = F::add(a, b);
+  y = F::sub(a, b);
+}
+
 }  // namespace sfn

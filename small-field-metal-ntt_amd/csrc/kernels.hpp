@@ -47,6 +47,7 @@ struct PassArgs {
   const uint32_t *post_m;         // LDE: per-block post-twist x n^-1 * sigma_b^low (blocks x 2^log_n)
   const uint32_t *pointf_m;       // LDE: per-block point factors sigma_b^(p << lo) (blocks x 2^LOG_T)
   uint32_t xcd_share_log;         // 2^s consecutive tiles share 128-byte lines: keep them on one XCD
+  uint32_t tab_low_major;         // twist_m / post_m entries ordered (low, p) instead of (p, low)
 };
 
 // V1 (one launch per layer) arguments.

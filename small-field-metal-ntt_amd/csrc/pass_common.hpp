@@ -115,6 +115,12 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
           // BB: twiddle depends on i mod 2^j -> i = m + q*2^(j+1)
           // M31: twiddle depends on i >> (j+1) -> i = (m << (j+1)) + q
           const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (F::ID == 1 && B0 == 0) {
+            if (m == 0) {  // position 0 of every layer: the twiddle is 1
+              bfly_unit<F>(v[i], v[i + (1 << j)]);
+              continue;
+            }
+          }
           if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
           else bfly_ct<F>(v[i], v[i + (1 << j)], w);
         }
@@ -134,6 +140,12 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
 #pragma unroll
         for (int q = 0; q < (1 << (F::ID == 1 ? (K - 1 - j) : j)); ++q) {
           const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (F::ID == 1 && B0 == 0) {
+            if (m == 0) {
+              bfly_unit<F>(v[i], v[i + (1 << j)]);
+              continue;
+            }
+          }
           if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
           else bfly_ct<F>(v[i], v[i + (1 << j)], w);
         }

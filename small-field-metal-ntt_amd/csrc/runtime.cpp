@@ -147,12 +147,13 @@ sfn_status Context::bb_root_table(const uint2 **out) {
   return SFN_OK;
 }
 
-sfn_status Context::bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, const uint32_t **out) {
+sfn_status Context::bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, bool low_major, const uint32_t **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(lo, hi, (int)inverse);
+  auto key = std::make_tuple(lo, hi, (int)inverse, (int)low_major);
   auto it = bb_twist_m.find(key);
   if (it == bb_twist_m.end()) {
     auto t = host::to_monty_words(host::bb_twist_table(lo, hi, inverse));
+    if (low_major) t = host::low_major(t, lo, hi);
     DevBuf b;
     sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
     if (s) return s;
@@ -162,14 +163,22 @@ sfn_status Context::bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, con
   return SFN_OK;
 }
 
-sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
+sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, bool low_major,
                                 const uint32_t **post, const uint32_t **pointf) {
   std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(log_n, added_bits, shift_std, lo);
+  auto key = std::make_tuple(log_n, added_bits, shift_std, lo, (int)low_major);
   auto it = bb_lde_fast_tab.find(key);
   if (it == bb_lde_fast_tab.end()) {
     std::vector<uint32_t> po, pf;
     host::bb_lde_fast_tables(log_n, added_bits, shift_std, lo, po, pf);
+    if (low_major) {  // every block's post table in (low, p) order
+      const size_t n = (size_t)1 << log_n;
+      for (size_t b = 0; b < po.size() / n; ++b) {
+        std::vector<uint32_t> blk(po.begin() + b * n, po.begin() + (b + 1) * n);
+        blk = host::low_major(blk, lo, log_n);
+        std::copy(blk.begin(), blk.end(), po.begin() + b * n);
+      }
+    }
     DevBuf a, b;
     sfn_status s = upload_table(a, po.data(), po.size() * 4);
     if (s) return s;
@@ -519,7 +528,9 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
       a.twist = nullptr;
       if (p.lo > 0) {
         const uint32_t *tm;
-        if ((st0 = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, &tm))) return st0;
+        // lanes of one workgroup share their low row bits when columns run fastest
+        a.tab_low_major = p.order == kColFastest;
+        if ((st0 = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st0;
         a.twist_m = tm;
       }
     }
@@ -666,8 +677,10 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       const uint2 *root;
       const uint32_t *tim, *post, *pointf;
       if ((st = bb_root_table(&root))) return st;
-      if ((st = bb_twist_m_table(lo, log_n, true, &tim))) return st;
-      if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, &post, &pointf))) return st;
+      const bool low_major = q.order == kColFastest;
+      if ((st = bb_twist_m_table(lo, log_n, true, low_major, &tim))) return st;
+      if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, low_major, &post, &pointf))) return st;
+      a.tab_low_major = low_major;
       a.tw = root;
       a.twist_m = tim;
       a.post_m = post;

@@ -39,7 +39,7 @@ struct Schedule {
   uint32_t max_log_t = 10;  // largest LDS sub-transform
   uint32_t logr = 5;        // register radix
   uint32_t log_w = 4;       // lanes per workgroup (strided passes)
-  uint32_t mid_log_w = 4;   // lanes per workgroup of the LDE middle pass
+  uint32_t mid_log_w = 3;   // lanes per workgroup of the LDE middle pass
 };
 
 struct TuneKey {
@@ -69,8 +69,9 @@ class Context {
   std::map<std::tuple<uint32_t, uint32_t, int>, DevBuf> bb_twist;         // (lo, hi, inverse)
   std::map<std::tuple<uint32_t, uint32_t, uint32_t>, DevBuf> bb_lde_mid;  // (L, a, shift_std)
   std::map<std::pair<uint32_t, int>, DevBuf> m31_tw;                      // (L, inverse)
-  std::map<std::tuple<uint32_t, uint32_t, int>, DevBuf> bb_twist_m;       // (lo, hi, inverse), Montgomery words
-  std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::pair<DevBuf, DevBuf>> bb_lde_fast_tab;  // (L, a, shift, lo)
+  std::map<std::tuple<uint32_t, uint32_t, int, int>, DevBuf> bb_twist_m;  // (lo, hi, inverse, low_major), Montgomery words
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, int>, std::pair<DevBuf, DevBuf>>
+      bb_lde_fast_tab;  // (L, a, shift, lo, low_major)
   size_t table_bytes = 0;
 
   // scratch for the host-pointer API and V3 ping-pong
@@ -99,9 +100,10 @@ class Context {
   sfn_status bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);
   sfn_status bb_root_table(const uint2 **out);
-  sfn_status bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, const uint32_t **out);
-  sfn_status bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, const uint32_t **post,
-                         const uint32_t **pointf);
+  // low_major: entry (low << (hi - lo)) | p instead of (p << lo) | low
+  sfn_status bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, bool low_major, const uint32_t **out);
+  sfn_status bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, bool low_major,
+                         const uint32_t **post, const uint32_t **pointf);
   sfn_status bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
   sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
   sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);

@@ -69,6 +69,14 @@ std::vector<Pair> bb_lde_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t
   return t;
 }
 
+std::vector<uint32_t> low_major(const std::vector<uint32_t> &t, uint32_t lo, uint32_t hi) {
+  const uint32_t lt = hi - lo;
+  std::vector<uint32_t> r(t.size());
+  for (uint64_t p = 0; p < ((uint64_t)1 << lt); ++p)
+    for (uint64_t low = 0; low < ((uint64_t)1 << lo); ++low) r[(low << lt) | p] = t[(p << lo) | low];
+  return r;
+}
+
 std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t) {
   std::vector<uint32_t> m(t.size());
   for (size_t i = 0; i < t.size(); ++i) m[i] = bb_to_monty(t[i].w);

@@ -63,6 +63,9 @@ std::vector<Pair> bb_root_table(uint32_t log);
 // g_hi^(+-low * bitrev(p, hi - lo)).
 std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse);
 
+// Reorders a table of 2^hi entries indexed (p << lo) | low into (low << (hi - lo)) | p.
+std::vector<uint32_t> low_major(const std::vector<uint32_t> &t, uint32_t lo, uint32_t hi);
+
 // Single-word Montgomery form (w * 2^32 mod p) of a table of standard values.
 std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t);
 
