@@ -46,6 +46,27 @@ __device__ __forceinline__ void bstore(uint32_t v, Rsrc r, uint32_t voff, uint32
   __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
 }
 
+// v[i] *= tab[off + i * step] for the R values of an item (Montgomery table words).
+template <int R>
+__device__ __forceinline__ void mul_table(uint32_t (&v)[R], Rsrc tab, uint32_t off_bytes, uint32_t step_bytes) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(tab, off_bytes, i * step_bytes));
+}
+
+// Same with the table staged in LDS, indexed by the point alone.
+template <int R, int B0>
+__device__ __forceinline__ void mul_lds_table(uint32_t (&v)[R], const uint32_t *tab, uint32_t pbase) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], tab[pbase | (i << B0)]);
+}
+
+// Copies T words from global to LDS with 16-byte accesses (T >= 4 * blockDim
+// is not required: threads loop).
+__device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *src, uint32_t T) {
+  for (uint32_t e = threadIdx.x * 4; e < T; e += blockDim.x * 4)
+    *reinterpret_cast<uint4 *>(dst + e) = *reinterpret_cast<const uint4 *>(src + e);
+}
+
 // BabyBear twiddles from the LDS root table (Shoup pairs).
 template <int LOG_T, int DIR>
 struct RootTw {
@@ -82,6 +103,10 @@ struct RoundIO {
   uint32_t sps, dps;        // elements between consecutive points
   uint32_t lo;
   uint32_t tps, tls;  // twist / post entry of (p, low) = (p << tps) + (low << tls)
+  uint32_t low0;      // low row bits of lane 0
+  // LDS copies of this workgroup's twist / post tables, indexed by the point
+  // (LDE middle pass with column-fastest lanes, which all share low0)
+  const uint32_t *ltw, *lpost;
   bool has_twist;
   uint32_t has_scale, scale_w, scale_q;
   // LDE blocks: per-point factor table (one block) and post-twist table (one block)
@@ -91,7 +116,7 @@ struct RoundIO {
 
 // Round r of a pass: loads from HBM when FIRST, else from LDS; stores to HBM
 // when LAST, else to LDS.  Items: W << (LOG_T - K).
-template <class F, int DIR, int LOG_T, bool CONTIG, int K, int B0, bool FIRST, bool LAST, class TW>
+template <class F, int DIR, int LOG_T, bool CONTIG, int K, int B0, bool FIRST, bool LAST, bool LT, class TW>
 __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const LaneInfo *lanes, TW tw,
                                           const RoundIO &io) {
   constexpr int R = 1 << K;
@@ -130,10 +155,8 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
         }
         if constexpr (F::ID == 1 && DIR == 1) {
           if (io.has_twist) {
-            const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-            const uint32_t tstep = (4u << (B0 + io.tps));
-#pragma unroll
-            for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
+            if constexpr (LT) mul_lds_table<R, B0>(v, io.ltw, pbase);
+            else mul_table<R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
           }
         }
       } else {
@@ -149,17 +172,16 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     if constexpr (LAST) {
       if (!li.valid) continue;
       if constexpr (F::ID == 1 && DIR == 0) {
-        if (io.has_twist) {
-          const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-          const uint32_t tstep = (4u << (B0 + io.tps));
-#pragma unroll
-          for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.twist, toff, i * tstep));
-        }
+        const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
+        if (io.has_twist) mul_table<R>(v, io.twist, toff, 4u << (B0 + io.tps));
         if (io.has_post) {
-          const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-          const uint32_t tstep = (4u << (B0 + io.tps));
-#pragma unroll
-          for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(io.post, toff, i * tstep));
+          if constexpr (LT) {
+            // keep the 32 table reads after the butterflies: hoisting them
+            // above the round costs 32 live registers at the peak
+            __builtin_amdgcn_sched_barrier(0);
+            mul_lds_table<R, B0>(v, io.lpost, pbase);
+          }
+          else mul_table<R>(v, io.post, toff, 4u << (B0 + io.tps));
         }
       }
       if (io.has_scale) {
@@ -192,16 +214,16 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
   }
 }
 
-template <class F, int DIR, int LOG_T, bool CONTIG, int r, int rend, class TW>
+template <class F, int DIR, int LOG_T, bool CONTIG, int r, int rend, bool LT = false, class TW>
 __device__ __forceinline__ void rounds(uint32_t *tile, uint32_t log_w, const LaneInfo *lanes, const TW &tw,
                                        const RoundIO &io) {
   using S = Sched<LOG_T, KR>;
   if constexpr (r < rend) {
     constexpr int K = S::template k<DIR, r>();
     constexpr int B0 = S::template b0<DIR, r>();
-    run_round<F, DIR, LOG_T, CONTIG, K, B0, r == 0, r == S::NR - 1>(tile, log_w, lanes, tw, io);
+    run_round<F, DIR, LOG_T, CONTIG, K, B0, r == 0, r == S::NR - 1, LT>(tile, log_w, lanes, tw, io);
     if constexpr (r + 1 < rend) __syncthreads();
-    rounds<F, DIR, LOG_T, CONTIG, r + 1, rend>(tile, log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, r + 1, rend, LT>(tile, log_w, lanes, tw, io);
   }
 }
 
@@ -251,6 +273,8 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.sps = (uint32_t)(((uint64_t)1 << a.lo) * a.src_rs);
   io.dps = (uint32_t)(((uint64_t)1 << a.lo) * a.dst_rs);
   io.lo = a.lo;
+  io.low0 = l0.low;
+  io.ltw = io.lpost = nullptr;
   io.tps = a.tab_low_major ? 0u : a.lo;
   io.tls = a.tab_low_major ? (uint32_t)LOG_T : 0u;
   io.has_twist = a.twist_m != nullptr;
@@ -282,8 +306,12 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
 // twist x n^-1 x sigma_b^low) on store, where sigma_b = shift * zeta_b.
 // The inverse's last round and the forward's first round own the same points
 // (one item per thread), so the coefficients stay in registers across blocks.
-template <int LOG_T>
-__global__ __launch_bounds__(512) void k_mid_fast(const PassArgs a) {
+// LT: the workgroup's lanes share their low row bits (columns run fastest), so
+// the twist, point-factor and post tables it needs are T words each; they are
+// staged in LDS ([tables: 2T words] between the root table and the tile)
+// instead of being read per lane from global memory.
+template <int LOG_T, bool LT>
+__global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   using F = BabyBear;
   using S = Sched<LOG_T, KR>;
   static_assert(S::NR >= 2, "middle pass needs two rounds");
@@ -292,20 +320,26 @@ __global__ __launch_bounds__(512) void k_mid_fast(const PassArgs a) {
   static_assert(S::template k<1, S::NR - 1>() == K && S::template b0<1, S::NR - 1>() == B0, "inverse last round");
   static_assert(S::template k<0, 0>() == K && S::template b0<0, 0>() == B0, "forward first round");
   constexpr int R = 1 << K;
+  constexpr uint32_t T = 1u << LOG_T;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t log_w = a.log_w;
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
-  uint32_t *tile = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
+  uint32_t *tabs = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
+  uint32_t *tile = tabs + (LT ? 2 * T : 0);
   RoundIO io;
   setup<LOG_T>(a, lanes, io);
   load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
+  if constexpr (LT) {
+    stage_words(tabs, a.twist_m + ((uint64_t)io.low0 << LOG_T), T);
+    io.ltw = tabs;
+  }
   __syncthreads();
   RootTw<LOG_T, 1> twi;
   RootTw<LOG_T, 0> twf;
   twi.tab = twf.tab = reinterpret_cast<const uint2 *>(tabp);
   // inverse rounds 0 .. NR-2 (round 0 loads from HBM with the inverse twist)
-  rounds<F, 1, LOG_T, true, 0, S::NR - 1>(tile, log_w, lanes, twi, io);
+  rounds<F, 1, LOG_T, true, 0, S::NR - 1, LT>(tile, log_w, lanes, twi, io);
   __syncthreads();
   // inverse last round: one item per thread (blockDim == W << (LOG_T - K))
   const uint32_t W = 1u << log_w;
@@ -320,18 +354,30 @@ __global__ __launch_bounds__(512) void k_mid_fast(const PassArgs a) {
   io.has_twist = false;  // the forward twist is part of post_b
   io.has_post = true;
   for (uint32_t b = 0; b < a.blocks; ++b) {
-    const Rsrc pf = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
+    // point factors of block b, stored item-major: entry pbase * R + i is the
+    // factor of point pbase | (i << B0), so an item reads R consecutive words
     uint32_t v[R];
 #pragma unroll
-    for (int i = 0; i < R; ++i) v[i] = F::mul_mont(c[i], bload(pf, pbase * 4u, (uint32_t)(i << B0) * 4u));
+    for (int i = 0; i < R; ++i) v[i] = c[i];
+    if constexpr (LT) {
+      if (b > 0) __syncthreads();  // every thread is done with block b-1's post table
+      stage_words(tabs, a.pointf_m + ((uint64_t)b << LOG_T), T);
+      stage_words(tabs + T, a.post_m + b * n + ((uint64_t)io.low0 << LOG_T), T);
+      __syncthreads();
+      mul_lds_table<R, 0>(v, tabs + pbase * R, 0);
+      io.lpost = tabs + T;
+    } else {
+      const Rsrc pf = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
+      mul_table<R>(v, pf, pbase * (4u * R), 4u);
+      io.post = make_rsrc(a.post_m + b * n, n * 4);
+    }
     round_regs<F, 0, K, B0>(v, pbase, twf);
     __syncthreads();  // every thread is done reading the tile
 #pragma unroll
     for (int i = 0; i < R; ++i) tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)] = v[i];
     __syncthreads();
     io.dst = make_rsrc(a.dst + b * a.dst_block_off + io.dbase0, (a.dst_elems - b * a.dst_block_off - io.dbase0) * 4);
-    io.post = make_rsrc(a.post_m + b * n, n * 4);
-    rounds<F, 0, LOG_T, true, 1, S::NR>(tile, log_w, lanes, twf, io);
+    rounds<F, 0, LOG_T, true, 1, S::NR, LT>(tile, log_w, lanes, twf, io);
   }
 }
 
@@ -393,19 +439,29 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
   return dir == 0 ? launch_fast_bb_fwd(log_t, contig, a, tiles, s) : launch_fast_bb_inv(log_t, contig, a, tiles, s);
 }
 
+template <int LOG_T, bool LT>
+static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t lds, hipStream_t s) {
+  static bool configured = false;
+  if (!configured) {
+    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxLdsBytes);
+    configured = true;
+  }
+  const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
+  hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  return hipGetLastError();
+}
+
+// LDS-staged tables when the lanes share their low row bits (tab_low_major)
+// and the extra 2T words fit; otherwise per-lane global table reads.
 template <int LOG_T>
 static hipError_t launch_mid_fast_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
   const size_t lds = fast::lds_bytes(1, LOG_T, a.log_w);
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
   if (lds > kMaxLdsBytes || threads > 512 || tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
-  static bool configured = false;
-  if (!configured) {
-    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxLdsBytes);
-    configured = true;
-  }
-  hipLaunchKernelGGL(fast::k_mid_fast<LOG_T>, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
-  return hipGetLastError();
+  const size_t lds_lt = lds + ((size_t)8 << LOG_T);
+  if (a.tab_low_major && lds_lt <= kMaxLdsBytes) return launch_mid_fast_lt<LOG_T, true>(a, tiles, lds_lt, s);
+  return launch_mid_fast_lt<LOG_T, false>(a, tiles, lds, s);
 }
 
 hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s) {

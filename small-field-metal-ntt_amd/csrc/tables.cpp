@@ -110,7 +110,8 @@ void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std,
     const uint32_t step = powmod(sigma, (uint64_t)1 << lo, kBBP);  // sigma^(2^lo)
     uint32_t f = 1;
     for (size_t p = 0; p < T; ++p) {
-      pointf[b * T + p] = bb_to_monty(f);
+      const size_t at = lt >= 5 ? ((p & ((T >> 5) - 1)) << 5) | (p >> (lt - 5)) : p;
+      pointf[b * T + at] = bb_to_monty(f);
       f = mulmod(f, step, kBBP);
     }
   }

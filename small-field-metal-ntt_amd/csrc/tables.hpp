@@ -71,7 +71,9 @@ std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t);
 
 // Coset-LDE tuned middle pass, sigma_b = shift * g_{L+a}^bitrev(b, a):
 //  post:   block b entry (p << lo) | low = g_L^(low*bitrev(p, L-lo)) * n^-1 * sigma_b^low
-//  pointf: block b entry p (p < 2^(L-lo)) = sigma_b^(p << lo)
+//  pointf: block b holds sigma_b^(p << lo) for p < 2^(L-lo), item-major for
+//          radix-32 items (L - lo >= 5): point p = r | (i << (L-lo-5)),
+//          r < 2^(L-lo-5), i < 32, sits at entry r * 32 + i
 // both in Montgomery form.
 void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
                         std::vector<uint32_t> &post, std::vector<uint32_t> &pointf);
