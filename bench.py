@@ -38,13 +38,22 @@ def kernel_bytes(label: str, log_n: int, batch: int, added_bits: int) -> int:
     return 8 * N        # DFT low layers over the extended columns
 
 
-def read_traffic(label_prefix: str):
+def traffic_key(label: str) -> str:
+    """Kernel label (sfn_pass_label) -> key of the counters summary:
+    k_mid_fast, k_fast_inv, k_fast_fwd, ..."""
+    head = label.split("<")[0]
+    if head == "k_fast" or head == "k_pass":
+        return head + ("_inv" if ",inv," in label else "_fwd")
+    return head
+
+
+def read_traffic(key: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     if not os.path.exists(COUNTERS_FILE):
         return None
     for line in open(COUNTERS_FILE):
         parts = line.split()
-        if parts and parts[0] == label_prefix and len(parts) >= 2:
+        if parts and parts[0] == key and len(parts) >= 2:
             kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
             if "hbm_bytes" in kv:
                 return float(kv["hbm_bytes"])
@@ -233,7 +242,7 @@ def main() -> int:
     dom = max(kernels, key=kernels.get)
     dom_bytes = kernel_bytes(dom, log_n, B, a)
     achieved = dom_bytes / (kernels[dom] * 1e-6) / 1e9
-    traffic = read_traffic(dom.split("<")[0])
+    traffic = read_traffic(traffic_key(dom))
     lde_bytes = 4 * B * (n + N)
     pipe_gbs = lde_bytes / (ms_per_step * 1e-3) / 1e9
 
