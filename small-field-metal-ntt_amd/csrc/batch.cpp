@@ -18,10 +18,8 @@ using sfn::SegDesc;
 struct sfn_plan {
   Context *ctx = nullptr;
   struct Launch {
-    std::vector<SegDesc> segs;
+    sfn::MultiLaunch m;
     void *d_segs = nullptr;
-    uint64_t tiles = 0;
-    size_t lds = 0;
     uint32_t grid = 0;
   };
   std::vector<Launch> launches;
@@ -41,7 +39,7 @@ struct sfn_plan {
 
   hipError_t issue(hipStream_t s) {
     for (auto &l : launches) {
-      hipError_t e = sfn::launch_multi((const SegDesc *)l.d_segs, (uint32_t)l.segs.size(), l.tiles, l.lds, l.grid, s);
+      hipError_t e = sfn::launch_multi_any(l.m, (const SegDesc *)l.d_segs, l.grid, s);
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -57,6 +55,7 @@ sfn_status sfn_plan_create(sfn_ctx *cx, const sfn_job_desc *jobs, uint32_t count
   (void)hipSetDevice(c.device);
   auto plan = std::make_unique<sfn_plan>();
   plan->ctx = &c;
+  std::vector<sfn::MultiJob> mj;
   for (uint32_t j = 0; j < count; ++j) {
     const sfn_job_desc &jd = jobs[j];
     sfn_status st = sfn::validate_ntt(&c, jd.field, SFN_VARIANT_AUTO, jd.dir, jd.log_n, jd.batch, SFN_LAYOUT_SOA);
@@ -66,38 +65,24 @@ sfn_status sfn_plan_create(sfn_ctx *cx, const sfn_job_desc *jobs, uint32_t count
       if (jd.d_in != jd.d_out) return c.fail(SFN_ERR_UNSUPPORTED, "out-of-place log_n == 0 job in a plan");
       continue;  // identity in place
     }
-    Schedule sc = c.schedule_for(jd.field, jd.log_n, jd.batch, SFN_LAYOUT_SOA);
-    auto passes = sfn::plan_ntt_passes(jd.field, SFN_VARIANT_AUTO, jd.dir, jd.log_n, jd.batch, SFN_LAYOUT_SOA, sc);
-    const uint64_t n = 1ull << jd.log_n;
-    const uint32_t *src = jd.d_in;
-    for (size_t k = 0; k < passes.size(); ++k) {
-      if (plan->launches.size() <= k) plan->launches.resize(k + 1);
-      SegDesc d{};
-      st = sfn::make_pass_args(&c, jd.field, jd.dir, passes[k], src, jd.d_out, n, 1, n, 1, jd.batch, jd.log_n, &d.a);
-      if (st) return st;
-      d.kind = (uint32_t)(jd.field * 2 + jd.dir);
-      d.log_t = (uint32_t)passes[k].log_t;
-      auto &L = plan->launches[k];
-      d.tile_begin = L.tiles;
-      L.tiles += passes[k].tiles;
-      L.lds = std::max(L.lds, sfn::pass_lds_bytes(jd.field, passes[k].log_t, passes[k].log_w));
-      L.segs.push_back(d);
-      src = jd.d_out;
-    }
+    mj.push_back(sfn::MultiJob{jd.field, jd.dir, jd.log_n, jd.batch, jd.d_in, jd.d_out});
   }
-  const uint32_t cus = (uint32_t)std::max(1, c.props.multiProcessorCount);
-  for (auto &L : plan->launches) {
-    // persistent grid: enough workgroups to fill every CU a few times over
-    const uint32_t per_cu = L.lds > 80 * 1024 ? 1 : (L.lds > 40 * 1024 ? 2 : 4);
-    L.grid = (uint32_t)std::min<uint64_t>(L.tiles, (uint64_t)cus * per_cu * 2);
-    const size_t bytes = L.segs.size() * sizeof(SegDesc);
+  std::vector<sfn::MultiLaunch> ml;
+  sfn_status bst = sfn::build_multi(&c, mj, ml);
+  if (bst) return bst;
+  for (auto &m : ml) {
+    sfn_plan::Launch L;
+    L.m = std::move(m);
+    L.grid = sfn::multi_grid(&c, L.m);
+    const size_t bytes = L.m.segs.size() * sizeof(SegDesc);
     if (bytes) {
       hipError_t e = hipMalloc(&L.d_segs, bytes);
       if (e != hipSuccess) return c.hip_fail(e, "plan descriptors");
-      e = hipMemcpy(L.d_segs, L.segs.data(), bytes, hipMemcpyHostToDevice);
+      e = hipMemcpy(L.d_segs, L.m.segs.data(), bytes, hipMemcpyHostToDevice);
       if (e != hipSuccess) return c.hip_fail(e, "plan descriptors");
     }
-    plan->total_tiles += L.tiles;
+    plan->total_tiles += L.m.tiles;
+    plan->launches.push_back(std::move(L));
   }
   if (use_graph) {
     hipError_t e = hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking);

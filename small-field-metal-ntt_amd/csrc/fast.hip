@@ -258,9 +258,9 @@ __device__ __forceinline__ void load_root(const PassArgs &a, uint2 *tab) {
 // Common workgroup set-up: lane records in LDS, buffer resources rebased at
 // lane 0 (decoded by every thread, so the bases stay scalar).
 template <int LOG_T>
-__device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundIO &io) {
+__device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundIO &io, uint32_t tile) {
   const uint32_t log_w = a.log_w;
-  const uint64_t g0 = (uint64_t)xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log) << log_w;
+  const uint64_t g0 = (uint64_t)tile << log_w;
   for (uint32_t l = threadIdx.x; l < (1u << log_w); l += blockDim.x) decode_lane(a, LOG_T, g0 + l, lanes[l]);
   LaneInfo l0;
   decode_lane(a, LOG_T, g0, l0);
@@ -285,19 +285,25 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.has_pointf = io.has_post = false;
 }
 
+// One tile (W lanes) of a tuned pass; any blockDim (threads loop over items).
 template <class F, int DIR, int LOG_T, bool CONTIG>
-__global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, unsigned char *smem) {
   const uint32_t log_w = a.log_w;
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
   uint32_t *tile = reinterpret_cast<uint32_t *>(tabp + (F::ID == 1 ? ((size_t)8 << LOG_T) : 0));
   RoundIO io;
-  setup<LOG_T>(a, lanes, io);
+  setup<LOG_T>(a, lanes, io, tile_id);
   if constexpr (F::ID == 1) load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   __syncthreads();
   const auto tw = make_tw<F, DIR, LOG_T>(a, tabp);
   rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
+}
+
+template <class F, int DIR, int LOG_T, bool CONTIG>
+__global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  fast_tile<F, DIR, LOG_T, CONTIG>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), smem);
 }
 
 // ---- coset LDE middle pass ----
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   uint32_t *tabs = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
   uint32_t *tile = tabs + (LT ? 2 * T : 0);
   RoundIO io;
-  setup<LOG_T>(a, lanes, io);
+  setup<LOG_T>(a, lanes, io, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log));
   load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   if constexpr (LT) {
     stage_words(tabs, a.twist_m + ((uint64_t)io.low0 << LOG_T), T);
@@ -477,5 +483,76 @@ hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hip
   }
 }
 #endif  // SFN_IN_PART(4)
+
+#if SFN_IN_PART(5)
+// ---- many jobs in one launch, tuned tiles ----
+// Segments (one per job pass) own consecutive ranges of a global tile index.
+// A persistent grid (a multiple of 8 workgroups) walks the tiles with a grid
+// stride, so every wave reaches the end of the list and the launch drains.
+// Global tile ids are permuted within groups of 8 * 2^xs like xcd_tile, so
+// tiles that share 128-byte lines run on one XCD.  Field, direction, tile size
+// and lane order are selected per segment (uniform across the workgroup).
+namespace fast {
+template <class F, int DIR, bool CONTIG>
+__device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, uint32_t tile, unsigned char *smem) {
+  switch (log_t) {
+#define SFN_CASE(L) \
+  case L: fast_tile<F, DIR, L, CONTIG>(a, tile, smem); break;
+    SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10) SFN_CASE(11)
+#undef SFN_CASE
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc *segs, uint32_t nseg,
+                                                                  uint64_t total_tiles, uint32_t xs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint64_t gs = 8ull << xs;
+  const uint64_t full = total_tiles - total_tiles % gs;
+  for (uint64_t t0 = blockIdx.x; t0 < total_tiles; t0 += gridDim.x) {
+    uint64_t t = t0;
+    if (xs && t0 < full) {
+      const uint64_t r = t0 & (gs - 1);
+      t = (t0 - r) + ((r & 7u) << xs) + (r >> 3);
+    }
+    uint32_t lo = 0, hi = nseg;  // segs[lo].tile_begin <= t < segs[hi].tile_begin
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (segs[mid].tile_begin <= t) lo = mid;
+      else hi = mid;
+    }
+    const PassArgs a = segs[lo].a;
+    const uint32_t local = (uint32_t)(t - segs[lo].tile_begin);
+    const uint32_t log_t = segs[lo].log_t;
+    switch (segs[lo].kind) {  // field * 2 + dir, + 4 when lanes are adjacent in memory
+      case 0: multi_tile<M31, 0, false>(log_t, a, local, smem); break;
+      case 1: multi_tile<M31, 1, false>(log_t, a, local, smem); break;
+      case 2: multi_tile<BabyBear, 0, false>(log_t, a, local, smem); break;
+      case 3: multi_tile<BabyBear, 1, false>(log_t, a, local, smem); break;
+      case 4: multi_tile<M31, 0, true>(log_t, a, local, smem); break;
+      case 5: multi_tile<M31, 1, true>(log_t, a, local, smem); break;
+      case 6: multi_tile<BabyBear, 0, true>(log_t, a, local, smem); break;
+      case 7: multi_tile<BabyBear, 1, true>(log_t, a, local, smem); break;
+      default: break;
+    }
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+}  // namespace fast
+
+hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
+                             uint32_t xs, hipStream_t s) {
+  if (total_tiles == 0) return hipSuccess;
+  if (lds > kMaxLdsBytes || grid == 0 || xs > 3) return hipErrorInvalidConfiguration;
+  static bool configured = false;
+  if (!configured) {
+    (void)hipFuncSetAttribute((const void *)fast::k_multi_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxLdsBytes);
+    configured = true;
+  }
+  hipLaunchKernelGGL(fast::k_multi_fast, dim3(grid), dim3(kMultiFastThreads), lds, s, d_segs, nseg, total_tiles, xs);
+  return hipGetLastError();
+}
+#endif  // SFN_IN_PART(5)
 
 }  // namespace sfn

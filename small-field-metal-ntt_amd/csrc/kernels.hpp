@@ -97,6 +97,11 @@ size_t fast_lds_bytes(int field, int log_t, uint32_t log_w);
 hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArgs &a, uint64_t tiles,
                        hipStream_t s);
 hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s);
+// Tuned multi-job pass: segments' kind = field * 2 + dir (+ 4 when lanes are
+// adjacent in memory), log_t in [6, 11]; kMultiFastThreads threads per workgroup.
+constexpr int kMultiFastThreads = 256;
+hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
+                             uint32_t xs, hipStream_t s);
 hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);
 
 }  // namespace sfn

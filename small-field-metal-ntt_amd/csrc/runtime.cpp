@@ -467,6 +467,29 @@ static bool fast_offsets_fit(const PassDesc &p, Layout ls, Layout ld) {
   return span(ls) < 0xFFFFFFFFull && span(ld) < 0xFFFFFFFFull;
 }
 
+// Tuned-kernel fields of a pass's argument block (after make_pass_args):
+// buffer extents, and for BabyBear the root table and the Montgomery twist.
+static sfn_status fill_fast_args(Context *ctx, int field, int dir, const PassDesc &p, uint64_t src_elems,
+                                 uint64_t dst_elems, PassArgs *out) {
+  PassArgs &a = *out;
+  a.src_elems = src_elems;
+  a.dst_elems = dst_elems;
+  if (field != SFN_FIELD_BABYBEAR) return SFN_OK;
+  const uint2 *root;
+  sfn_status st = ctx->bb_root_table(&root);
+  if (st) return st;
+  a.tw = root;
+  a.twist = nullptr;
+  if (p.lo > 0) {
+    const uint32_t *tm;
+    // lanes of one workgroup share their low row bits when columns run fastest
+    a.tab_low_major = p.order == kColFastest;
+    if ((st = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st;
+    a.twist_m = tm;
+  }
+  return SFN_OK;
+}
+
 static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src,
                               uint32_t *dst, Layout ls, Layout ld, uint32_t batch, uint32_t log_n_real,
                               hipStream_t s, uint64_t src_elems = 0, uint64_t dst_elems = 0) {
@@ -518,22 +541,8 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
   if (p.fast && ctx->allow_fast && (p.lane_contig || aligned) && src_elems && dst_elems &&
       fast_offsets_fit(p, ls, ld)) {
-    a.src_elems = src_elems;
-    a.dst_elems = dst_elems;
     a.xcd_share_log = xcd_share_log(ctx, p);
-    if (field == SFN_FIELD_BABYBEAR) {
-      const uint2 *root;
-      if ((st0 = ctx->bb_root_table(&root))) return st0;
-      a.tw = root;
-      a.twist = nullptr;
-      if (p.lo > 0) {
-        const uint32_t *tm;
-        // lanes of one workgroup share their low row bits when columns run fastest
-        a.tab_low_major = p.order == kColFastest;
-        if ((st0 = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st0;
-        a.twist_m = tm;
-      }
-    }
+    if ((st0 = fill_fast_args(ctx, field, dir, p, src_elems, dst_elems, &a))) return st0;
     (void)ctx->timing_mark(s, true, std::string("k_fast<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
                                        std::to_string(p.log_t) + (p.lane_contig ? ",lanes" : ",points") +
                                        ">lo=" + std::to_string(p.lo));
@@ -717,6 +726,113 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     if (st) return st;
   }
   return SFN_OK;
+}
+
+// ------------------------------------------------------------------ multi-job launches
+namespace {
+
+// Lanes of a tuned multi-job tile (SoA columns): W * T = 8192 points, one
+// 32-point item per thread of a 256-thread workgroup.
+bool choose_lanes_multi(PassDesc &p, uint32_t batch) {
+  if (p.log_t < 6 || p.log_t > (int)kMaxLogTile || p.logr != 5) return false;
+  uint32_t lw = 13 - (uint32_t)p.log_t;
+  const uint64_t instances = (uint64_t)batch << (p.log_n - p.log_t);
+  while (lw > 0 && (1ull << (lw - 1)) >= instances) --lw;  // no wholly idle lane groups
+  if (p.lo > 0) {
+    p.order = kLowFastest;  // lanes: adjacent low rows of one column
+    p.lane_contig = 1;
+    lw = std::min<uint32_t>(lw, p.lo);
+  } else {
+    p.order = kHighFastest;  // lanes: contiguous sub-transforms (points adjacent)
+    p.lane_contig = 0;
+  }
+  if (fast_lds_bytes(1, p.log_t, lw) > kMaxLdsBytes / 2) return false;
+  p.log_w = lw;
+  p.fast = true;
+  p.tiles = (instances + (1ull << lw) - 1) >> lw;
+  return true;
+}
+
+}  // namespace
+
+sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vector<MultiLaunch> &out) {
+  out.clear();
+  std::vector<std::vector<PassDesc>> jp(jobs.size());
+  size_t nl = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const MultiJob &m = jobs[j];
+    if (m.log_n == 0) continue;
+    Schedule sc = ctx->schedule_for(m.field, m.log_n, m.batch, SFN_LAYOUT_SOA);
+    // many jobs fill the device together: prefer few, large passes unless tuned
+    if (ctx->tuned.find(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}) == ctx->tuned.end()) {
+      sc.max_log_t = kMaxLogTile;
+      sc.logr = 5;
+    }
+    jp[j] = plan_ntt_passes(m.field, SFN_VARIANT_AUTO, m.dir, m.log_n, m.batch, SFN_LAYOUT_SOA, sc, false);
+    nl = std::max(nl, jp[j].size());
+  }
+  out.resize(nl);
+  for (size_t k = 0; k < nl; ++k) {
+    MultiLaunch &L = out[k];
+    // tuned when every pass of this launch fits the tuned kernel
+    bool tuned = ctx->allow_fast;
+    std::vector<PassDesc> q(jobs.size());
+    for (size_t j = 0; j < jobs.size() && tuned; ++j) {
+      if (k >= jp[j].size()) continue;
+      const MultiJob &m = jobs[j];
+      q[j] = jp[j][k];
+      const Layout lay{(uint64_t)1 << m.log_n, 1};
+      const bool aligned = ((uintptr_t)m.src % 16 == 0) && ((uintptr_t)m.dst % 16 == 0);
+      tuned = choose_lanes_multi(q[j], m.batch) && (q[j].lane_contig || aligned) && fast_offsets_fit(q[j], lay, lay);
+    }
+    L.tuned = tuned;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (k >= jp[j].size()) continue;
+      const MultiJob &m = jobs[j];
+      const PassDesc &p = tuned ? q[j] : jp[j][k];
+      const uint64_t n = 1ull << m.log_n;
+      const uint32_t *src = k == 0 ? m.src : m.dst;  // later passes work in place on dst
+      SegDesc d{};
+      sfn_status st = make_pass_args(ctx, m.field, m.dir, p, src, m.dst, n, 1, n, 1, m.batch, m.log_n, &d.a);
+      if (st) return st;
+      d.kind = (uint32_t)(m.field * 2 + m.dir);
+      d.log_t = (uint32_t)p.log_t;
+      if (tuned) {
+        if ((st = fill_fast_args(ctx, m.field, m.dir, p, (uint64_t)m.batch << m.log_n, (uint64_t)m.batch << m.log_n,
+                                 &d.a)))
+          return st;
+        if (p.lane_contig) {
+          d.kind += 4;
+          if (p.log_w < 5) L.xs = std::max<uint32_t>(L.xs, std::min<uint32_t>(3, 5 - p.log_w));
+        }
+        L.lds = std::max(L.lds, fast_lds_bytes(m.field, p.log_t, p.log_w));
+      } else {
+        L.lds = std::max(L.lds, pass_lds_bytes(m.field, p.log_t, p.log_w));
+      }
+      d.tile_begin = L.tiles;
+      L.tiles += p.tiles;
+      L.segs.push_back(d);
+    }
+  }
+  return SFN_OK;
+}
+
+uint32_t multi_grid(const Context *ctx, const MultiLaunch &L) {
+  const uint64_t cus = (uint64_t)std::max(1, ctx->props.multiProcessorCount);
+  if (L.tuned) {
+    // resident workgroups per CU by LDS (at most 4 of 4 waves), a multiple of 8
+    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(4, kMaxLdsBytes / std::max<size_t>(L.lds, 1)));
+    const uint64_t cap = std::max<uint64_t>(8, (cus * per_cu) & ~7ull);
+    const uint64_t want = (L.tiles + 7) & ~7ull;
+    return (uint32_t)std::min(cap, want);
+  }
+  const uint32_t per_cu = L.lds > 80 * 1024 ? 1 : (L.lds > 40 * 1024 ? 2 : 4);
+  return (uint32_t)std::min<uint64_t>(L.tiles, cus * per_cu * 2);
+}
+
+hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s) {
+  if (L.tuned) return launch_multi_fast(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, L.xs, s);
+  return launch_multi(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, s);
 }
 
 }  // namespace sfn

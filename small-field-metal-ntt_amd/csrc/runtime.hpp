@@ -125,6 +125,27 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
 
 sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t log_n, uint32_t batch,
                         int layout);
+
+// ---- many transforms in one launch per pass (plans, server groups) ----
+struct MultiJob {
+  int field, dir;
+  uint32_t log_n, batch;  // SoA columns
+  const uint32_t *src;
+  uint32_t *dst;
+};
+struct MultiLaunch {
+  std::vector<SegDesc> segs;  // tile_begin ascending
+  uint64_t tiles = 0;
+  size_t lds = 0;
+  bool tuned = false;  // k_multi_fast (radix-32 tiles) instead of the generic k_multi
+  uint32_t xs = 0;     // XCD grouping of tiles (tuned launches)
+};
+// Pass k of every job goes into launch k.  A launch uses the tuned kernel when
+// every one of its passes fits it (tile 2^6..2^11, offsets, alignment).
+sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vector<MultiLaunch> &out);
+// Persistent grid for a launch on this device.
+uint32_t multi_grid(const Context *ctx, const MultiLaunch &L);
+hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s);
 sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout);
 
 }  // namespace sfn

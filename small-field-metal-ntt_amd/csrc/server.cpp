@@ -133,48 +133,26 @@ sfn_status sfn_server::run_group(std::vector<Job *> &g, hipStream_t s, sfn::DevB
     }
   } else {
     // one multi-job launch per pass
-    std::vector<std::vector<sfn::SegDesc>> L;
-    std::vector<uint64_t> tiles;
-    std::vector<size_t> lds;
+    std::vector<sfn::MultiJob> mj;
     for (size_t i = 0; i < g.size(); ++i) {
       const sfn_request &r = g[i]->req;
-      sfn::Schedule sc = c.schedule_for(r.field, r.log_n, r.batch, SFN_LAYOUT_SOA);
-      auto passes = sfn::plan_ntt_passes(r.field, SFN_VARIANT_AUTO, r.dir, r.log_n, r.batch, SFN_LAYOUT_SOA, sc);
-      const uint64_t n = 1ull << r.log_n;
-      const uint32_t *src = din[i];
-      for (size_t k = 0; k < passes.size(); ++k) {
-        if (L.size() <= k) {
-          L.resize(k + 1);
-          tiles.resize(k + 1, 0);
-          lds.resize(k + 1, 0);
-        }
-        sfn::SegDesc d{};
-        sfn_status st = sfn::make_pass_args(&c, r.field, r.dir, passes[k], src, dout[i], n, 1, n, 1, r.batch,
-                                            r.log_n, &d.a);
-        if (st) return st;
-        d.kind = (uint32_t)(r.field * 2 + r.dir);
-        d.log_t = (uint32_t)passes[k].log_t;
-        d.tile_begin = tiles[k];
-        tiles[k] += passes[k].tiles;
-        lds[k] = std::max(lds[k], sfn::pass_lds_bytes(r.field, passes[k].log_t, passes[k].log_w));
-        L[k].push_back(d);
-        src = dout[i];
-      }
+      mj.push_back(sfn::MultiJob{r.field, r.dir, r.log_n, r.batch, din[i], dout[i]});
     }
+    std::vector<sfn::MultiLaunch> ml;
+    sfn_status st = sfn::build_multi(&c, mj, ml);
+    if (st) return st;
     size_t total = 0;
-    for (auto &v : L) total += v.size() * sizeof(sfn::SegDesc);
+    for (auto &m : ml) total += m.segs.size() * sizeof(sfn::SegDesc);
     if (grow(&c, segs, total)) return SFN_ERR_BUFFER_ALLOC_FAILED;
     std::vector<sfn::SegDesc> flat;
-    for (auto &v : L) flat.insert(flat.end(), v.begin(), v.end());
+    for (auto &m : ml) flat.insert(flat.end(), m.segs.begin(), m.segs.end());
     e = hipMemcpyAsync(segs.ptr, flat.data(), total, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
-    const uint32_t cus = (uint32_t)std::max(1, c.props.multiProcessorCount);
     size_t off = 0;
-    for (size_t k = 0; k < L.size(); ++k) {
-      const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles[k], (uint64_t)cus * 4);
-      e = sfn::launch_multi((const sfn::SegDesc *)segs.ptr + off, (uint32_t)L[k].size(), tiles[k], lds[k], grid, s);
+    for (auto &m : ml) {
+      e = sfn::launch_multi_any(m, (const sfn::SegDesc *)segs.ptr + off, sfn::multi_grid(&c, m), s);
       if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
-      off += L[k].size();
+      off += m.segs.size();
     }
   }
   if (!r0.device_ptrs) {
