@@ -60,6 +60,19 @@ def read_traffic(key: str):
     return None
 
 
+def host_cpu() -> str:
+    """nproc and the CPU model of this host (recorded beside the CPU baselines)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{os.cpu_count()} x {model}"
+
+
 def cpu_baseline(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw: int,
                  gpu_first_cols=None) -> dict:
     """The oracle's restatement of coset_lde_batch_cpu (src/lde/mod.rs:127-176),
@@ -75,7 +88,7 @@ def cpu_baseline(x_soa_sample, log_n: int, cols: int, added_bits: int, shift_raw
     res = {"value": round(cols * (1 << (log_n + added_bits)) / dt / 1e6, 3), "unit": "Melem/s",
            "cores": 1, "kind": "port",
            "sample": f"{cols} of 256 columns, 2^{log_n} rows, {1 << added_bits}x, SoA, {dt:.2f} s, "
-                     f"C restatement of coset_lde_batch_cpu, 1 thread of {os.cpu_count()}"}
+                     f"C restatement of coset_lde_batch_cpu, 1 thread", "host": host_cpu()}
     if gpu_first_cols is not None:
         res["parity_sample"] = bool(np.array_equal(out, gpu_first_cols))
     return res
@@ -93,7 +106,8 @@ def cpu_baseline_all_cores(x_soa_sample, log_n: int, cols: int, added_bits: int,
     dt = time.perf_counter() - t0
     return {"value": round(cols * (1 << (log_n + added_bits)) / dt / 1e6, 3), "unit": "Melem/s",
             "cores": threads, "kind": "port",
-            "sample": f"{cols} of 256 columns, {dt:.2f} s, columns split over {threads} threads"}
+            "sample": f"{cols} of 256 columns, {dt:.2f} s, columns split over {threads} threads",
+            "host": host_cpu()}
 
 
 def end_to_end(ctx, x_dev, log_n: int, B: int, a: int, shift_raw: int, layout: int, steps: int) -> dict:
@@ -143,7 +157,8 @@ def cpu_baseline_ntt(workload: str) -> dict:
     orc.ntt_batch(field, False, log_n, cols, x, threads=1)
     dt = (time.perf_counter() - t0) / cols
     return {"value": round(n / dt / 1e6, 3), "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"{cols} columns of 2^{log_n} in one call, {dt * 1e6:.0f} us per column, 1 thread"}
+            "sample": f"{cols} columns of 2^{log_n} in one call, {dt * 1e6:.0f} us per column, 1 thread",
+            "host": host_cpu()}
 
 
 def main() -> int:

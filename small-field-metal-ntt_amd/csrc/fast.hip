@@ -336,9 +336,12 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   RoundIO io;
   setup<LOG_T>(a, lanes, io, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log));
   load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
+  // LT tables: [point factors of the current block: T words][post of the
+  // current block: T words]; the post half holds the inverse twist first.
   if constexpr (LT) {
-    stage_words(tabs, a.twist_m + ((uint64_t)io.low0 << LOG_T), T);
-    io.ltw = tabs;
+    stage_words(tabs + T, a.twist_m + ((uint64_t)io.low0 << LOG_T), T);
+    stage_words(tabs, a.pointf_m, T);
+    io.ltw = tabs + T;
   }
   __syncthreads();
   RootTw<LOG_T, 1> twi;
@@ -359,6 +362,18 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   const uint64_t n = 1ull << a.log_n;
   io.has_twist = false;  // the forward twist is part of post_b
   io.has_post = true;
+  // LT: the next block's tables are written between the two barriers of the
+  // current block's exchange (by then every thread has used the current point
+  // factors, and the previous post table); when one 16-byte word per thread
+  // covers a table they are fetched into registers a round ahead.
+  const uint32_t e4 = threadIdx.x * 4;
+  const bool pre = LT && blockDim.x * 4 >= T;
+  uint4 po = make_uint4(0, 0, 0, 0), pf = po;
+  const uint32_t *post_lane = a.post_m + ((uint64_t)io.low0 << LOG_T);
+  if (pre && e4 < T) {
+    po = *reinterpret_cast<const uint4 *>(post_lane + e4);
+    if (a.blocks > 1) pf = *reinterpret_cast<const uint4 *>(a.pointf_m + T + e4);
+  }
   for (uint32_t b = 0; b < a.blocks; ++b) {
     // point factors of block b, stored item-major: entry pbase * R + i is the
     // factor of point pbase | (i << B0), so an item reads R consecutive words
@@ -366,21 +381,30 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
 #pragma unroll
     for (int i = 0; i < R; ++i) v[i] = c[i];
     if constexpr (LT) {
-      if (b > 0) __syncthreads();  // every thread is done with block b-1's post table
-      stage_words(tabs, a.pointf_m + ((uint64_t)b << LOG_T), T);
-      stage_words(tabs + T, a.post_m + b * n + ((uint64_t)io.low0 << LOG_T), T);
-      __syncthreads();
       mul_lds_table<R, 0>(v, tabs + pbase * R, 0);
-      io.lpost = tabs + T;
     } else {
-      const Rsrc pf = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
-      mul_table<R>(v, pf, pbase * (4u * R), 4u);
+      const Rsrc pf_r = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
+      mul_table<R>(v, pf_r, pbase * (4u * R), 4u);
       io.post = make_rsrc(a.post_m + b * n, n * 4);
     }
     round_regs<F, 0, K, B0>(v, pbase, twf);
-    __syncthreads();  // every thread is done reading the tile
+    __syncthreads();  // every thread is done reading the tile (and the tables in use)
 #pragma unroll
     for (int i = 0; i < R; ++i) tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)] = v[i];
+    if constexpr (LT) {
+      if (pre) {
+        if (e4 < T) {
+          *reinterpret_cast<uint4 *>(tabs + T + e4) = po;
+          if (b + 1 < a.blocks) *reinterpret_cast<uint4 *>(tabs + e4) = pf;
+          if (b + 1 < a.blocks) po = *reinterpret_cast<const uint4 *>(post_lane + (b + 1) * n + e4);
+          if (b + 2 < a.blocks) pf = *reinterpret_cast<const uint4 *>(a.pointf_m + ((uint64_t)(b + 2) << LOG_T) + e4);
+        }
+      } else {
+        stage_words(tabs + T, post_lane + b * n, T);
+        if (b + 1 < a.blocks) stage_words(tabs, a.pointf_m + ((uint64_t)(b + 1) << LOG_T), T);
+      }
+      io.lpost = tabs + T;
+    }
     __syncthreads();
     io.dst = make_rsrc(a.dst + b * a.dst_block_off + io.dbase0, (a.dst_elems - b * a.dst_block_off - io.dbase0) * 4);
     rounds<F, 0, LOG_T, true, 1, S::NR, LT>(tile, log_w, lanes, twf, io);
