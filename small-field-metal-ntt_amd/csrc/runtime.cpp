@@ -577,7 +577,10 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
     return SFN_OK;
   }
   timing_begin_sequence(s);
-  Schedule sc = force ? *force : schedule_for(field, log_n, batch, layout);
+  // the autotuner's schedule is the AUTO variant's; V1-V4 keep their own tiling
+  Schedule sc = force ? *force
+                      : (variant == SFN_VARIANT_AUTO ? schedule_for(field, log_n, batch, layout)
+                                                     : schedule_for(-1, log_n, batch, layout));
   const bool fwd = dir == SFN_FORWARD;
   std::vector<PassDesc> passes = plan_ntt_passes(field, variant, dir, log_n, batch, layout, sc, true);
   (void)fwd;

@@ -108,6 +108,8 @@ class Context {
   sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
   sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);
 
+  // Autotuned schedule of (field, log_n, batch, layout) if one is stored (field 2 =
+  // the LDE), else the default for the size; field -1 always gives the default.
   Schedule schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const;
 
   // Plans and runs a transform on device buffers.

@@ -4,6 +4,7 @@ bb16    BabyBear forward NTT, one column of 2^16, every variant
         (benches/ntt_benchmark.rs:632-700 shootout shape, seed 12345 + n)
 m31_20  M31 circle forward NTT, one column of 2^20, every variant, twiddles
         resident on the device (benches/ntt_benchmark.rs:183-375 shape)
+        ("auto" is the autotuner's schedule for the shape, tuned before timing)
 uneven  512 BabyBear/M31 columns with log_n ~ U{10..22} in one multi-job plan
         (one launch per pass, hipGraph replay) against launching every column
         on its own
@@ -34,6 +35,9 @@ def _single(ctx, field: int, log_n: int, steps: int, warmup: int) -> dict:
     din, dout = sfntt.DeviceBuffer(ctx, n), sfntt.DeviceBuffer(ctx, n)
     din.upload(x)
     ctx.twiddles_prepare(field, log_n)
+    # AUTO runs the autotuner's schedule for this shape (tuning is not timed)
+    best = ctypes.c_uint32()
+    ctx.check(lib.sfn_autotune(ctx.ptr, field, log_n, 1, L.LAYOUT_SOA, ctypes.byref(best)))
     res = {}
     ns = ctypes.c_uint64()
     for name, v in VARIANTS:

@@ -1,0 +1,100 @@
+"""GPU: every kernel path gives the same bits.
+
+The library picks among several kernel families for one transform (tuned
+radix-32 tiles, generic LDS-staged rounds, the XCD-grouped tile order, the
+autotuner's schedules).  Each switch is a performance choice only: these tests
+run the same inputs through a context with the switch off and compare the
+outputs with each other and with the CPU oracle.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sfntt
+from sfntt import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+M31, BB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR
+
+
+def context_with(env: dict):
+    """A context created while `env` is set (the switches are read at creation)."""
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return sfntt.HipContext(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return sfntt.HipContext(0)
+
+
+CASES_NTT = [(M31, 12, 1), (M31, 17, 3), (M31, 20, 1), (BB, 11, 5), (BB, 16, 4), (BB, 20, 2)]
+CASES_LDE = [(10, 24, 1), (14, 16, 2), (16, 8, 1)]
+
+
+def _run_all(c):
+    out = []
+    for field, log_n, batch in CASES_NTT:
+        x = orc.lcg_fill(1000 + log_n, batch << log_n, field)
+        for layout in (sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR):
+            for d in (sfntt.FORWARD, sfntt.INVERSE):
+                print(f"[paths] ntt field={field} log_n={log_n} batch={batch} layout={layout} dir={d}", flush=True)
+                y, _ = c.ntt(field, sfntt.VARIANT_AUTO, d, log_n, batch, x, layout)
+                out.append(y)
+    for log_n, batch, a in CASES_LDE:
+        x = orc.lcg_fill(42 + log_n, batch << log_n, BB)
+        for layout, shift in ((sfntt.LAYOUT_SOA, 0), (sfntt.LAYOUT_ROW_MAJOR, orc.bb_to_monty(31))):
+            print(f"[paths] lde log_n={log_n} batch={batch} a={a} layout={layout}", flush=True)
+            y, _ = c.coset_lde(x, log_n, batch, a, shift, layout)
+            out.append(y)
+    return out
+
+
+@pytest.fixture(scope="module")
+def reference_outputs(ctx):
+    outs = _run_all(ctx)
+    # anchor the default path on the oracle for a few of the cases
+    x = orc.lcg_fill(1000 + 16, 4 << 16, BB)
+    exp = orc.ntt_batch(BB, False, 16, 4, x)
+    assert np.array_equal(outs[4 * 4], exp), "default path differs from the oracle"
+    x = orc.lcg_fill(42 + 14, 16 << 14, BB)
+    assert np.array_equal(outs[4 * len(CASES_NTT) + 2], orc.coset_lde_batch(x, 14, 16, 2)), "default LDE differs"
+    return outs
+
+
+@pytest.mark.parametrize("env", [{"SFNTT_GENERIC_ONLY": "1"}, {"SFNTT_XCD_REMAP": "0"}],
+                         ids=["generic_kernels", "xcd_order_off"])
+def test_switch_gives_same_bits(reference_outputs, env):
+    c = context_with(env)
+    got = _run_all(c)
+    assert len(got) == len(reference_outputs)
+    for i, (g, r) in enumerate(zip(got, reference_outputs)):
+        assert np.array_equal(g, r), f"case {i} differs with {env}"
+    c.close()
+
+
+def test_autotuned_schedules_give_same_bits(ctx, reference_outputs):
+    c = sfntt.HipContext(0)
+    best = ctypes.c_uint32()
+    for field, log_n, batch in CASES_NTT:
+        for layout in (sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR):
+            c.check(c._lib.sfn_autotune(c.ptr, field, log_n, batch, layout, ctypes.byref(best)))
+    for log_n, batch, _ in CASES_LDE:  # field 2 = the LDE (2x)
+        for layout in (sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR):
+            c.check(c._lib.sfn_autotune(c.ptr, 2, log_n, batch, layout, ctypes.byref(best)))
+    got = _run_all(c)
+    for i, (g, r) in enumerate(zip(got, reference_outputs)):
+        assert np.array_equal(g, r), f"case {i} differs under autotuned schedules"
+    c.close()
