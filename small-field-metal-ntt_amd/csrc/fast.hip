@@ -286,24 +286,37 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
 }
 
 // One tile (W lanes) of a tuned pass; any blockDim (threads loop over items).
-template <class F, int DIR, int LOG_T, bool CONTIG>
-__device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, unsigned char *smem) {
-  const uint32_t log_w = a.log_w;
-  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
-  unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
-  uint32_t *tile = reinterpret_cast<uint32_t *>(tabp + (F::ID == 1 ? ((size_t)8 << LOG_T) : 0));
+// BabyBear twiddles come from an LDS root table of 2^ROOT_LOG entries
+// (g_{2^ROOT_LOG}^e): the twiddle of layer t, position j is entry
+// j << (ROOT_LOG - t - 1) whatever the sub-transform size, so one table of
+// 2^11 entries serves every tile size.  The caller fills it before the call
+// (the barrier after the lane set-up publishes it).
+template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG>
+__device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, LaneInfo *lanes, const uint2 *root,
+                                          uint32_t *tile) {
+  static_assert(ROOT_LOG >= LOG_T, "root table too small");
   RoundIO io;
   setup<LOG_T>(a, lanes, io, tile_id);
-  if constexpr (F::ID == 1) load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   __syncthreads();
-  const auto tw = make_tw<F, DIR, LOG_T>(a, tabp);
-  rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
+  if constexpr (F::ID == 1) {
+    RootTw<ROOT_LOG, DIR> tw;
+    tw.tab = root;
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, a.log_w, lanes, tw, io);
+  } else {
+    const auto tw = make_tw<F, DIR, LOG_T>(a, nullptr);
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, a.log_w, lanes, tw, io);
+  }
 }
 
 template <class F, int DIR, int LOG_T, bool CONTIG>
 __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fast_tile<F, DIR, LOG_T, CONTIG>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), smem);
+  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
+  uint2 *root = reinterpret_cast<uint2 *>(smem + (kLaneInfoBytes << a.log_w));
+  uint32_t *tile = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(root) +
+                                                (F::ID == 1 ? ((size_t)8 << LOG_T) : 0));
+  if constexpr (F::ID == 1) load_root<LOG_T>(a, root);
+  fast_tile<F, DIR, LOG_T, CONTIG, LOG_T>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
 }
 
 // ---- coset LDE middle pass ----
@@ -518,19 +531,29 @@ hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hip
 // and lane order are selected per segment (uniform across the workgroup).
 namespace fast {
 template <class F, int DIR, bool CONTIG>
-__device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, uint32_t tile, unsigned char *smem) {
+__device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, uint32_t tile, LaneInfo *lanes,
+                                           const uint2 *root, uint32_t *tile_lds) {
   switch (log_t) {
 #define SFN_CASE(L) \
-  case L: fast_tile<F, DIR, L, CONTIG>(a, tile, smem); break;
+  case L: fast_tile<F, DIR, L, CONTIG, kMaxLogTile>(a, tile, lanes, root, tile_lds); break;
     SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10) SFN_CASE(11)
 #undef SFN_CASE
     default: break;
   }
 }
 
+// LDS: [BabyBear root table, 2^11 Shoup pairs, loaded once][lane records:
+// lanes_bytes][tile]
 __global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc *segs, uint32_t nseg,
-                                                                  uint64_t total_tiles, uint32_t xs) {
+                                                                  uint64_t total_tiles, uint32_t xs,
+                                                                  const uint2 *root_g, uint32_t lanes_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
+  uint2 *root = reinterpret_cast<uint2 *>(smem);
+  const size_t root_bytes = root_g ? ((size_t)8 << kMaxLogTile) : 0;
+  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem + root_bytes);
+  uint32_t *tile_lds = reinterpret_cast<uint32_t *>(smem + root_bytes + lanes_bytes);
+  if (root_g)
+    for (uint32_t e = threadIdx.x; e < (1u << kMaxLogTile); e += blockDim.x) root[e] = root_g[e];
   const uint64_t gs = 8ull << xs;
   const uint64_t full = total_tiles - total_tiles % gs;
   for (uint64_t t0 = blockIdx.x; t0 < total_tiles; t0 += gridDim.x) {
@@ -549,14 +572,14 @@ __global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc 
     const uint32_t local = (uint32_t)(t - segs[lo].tile_begin);
     const uint32_t log_t = segs[lo].log_t;
     switch (segs[lo].kind) {  // field * 2 + dir, + 4 when lanes are adjacent in memory
-      case 0: multi_tile<M31, 0, false>(log_t, a, local, smem); break;
-      case 1: multi_tile<M31, 1, false>(log_t, a, local, smem); break;
-      case 2: multi_tile<BabyBear, 0, false>(log_t, a, local, smem); break;
-      case 3: multi_tile<BabyBear, 1, false>(log_t, a, local, smem); break;
-      case 4: multi_tile<M31, 0, true>(log_t, a, local, smem); break;
-      case 5: multi_tile<M31, 1, true>(log_t, a, local, smem); break;
-      case 6: multi_tile<BabyBear, 0, true>(log_t, a, local, smem); break;
-      case 7: multi_tile<BabyBear, 1, true>(log_t, a, local, smem); break;
+      case 0: multi_tile<M31, 0, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 1: multi_tile<M31, 1, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 2: multi_tile<BabyBear, 0, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 3: multi_tile<BabyBear, 1, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 4: multi_tile<M31, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 5: multi_tile<M31, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 6: multi_tile<BabyBear, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 7: multi_tile<BabyBear, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
       default: break;
     }
     __syncthreads();  // the next tile reuses the LDS
@@ -565,7 +588,7 @@ __global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc 
 }  // namespace fast
 
 hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
-                             uint32_t xs, hipStream_t s) {
+                             uint32_t xs, const void *bb_root, uint32_t lanes_bytes, hipStream_t s) {
   if (total_tiles == 0) return hipSuccess;
   if (lds > kMaxLdsBytes || grid == 0 || xs > 3) return hipErrorInvalidConfiguration;
   static bool configured = false;
@@ -574,7 +597,8 @@ hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t tota
                               (int)kMaxLdsBytes);
     configured = true;
   }
-  hipLaunchKernelGGL(fast::k_multi_fast, dim3(grid), dim3(kMultiFastThreads), lds, s, d_segs, nseg, total_tiles, xs);
+  hipLaunchKernelGGL(fast::k_multi_fast, dim3(grid), dim3(kMultiFastThreads), lds, s, d_segs, nseg, total_tiles, xs,
+                     reinterpret_cast<const uint2 *>(bb_root), lanes_bytes);
   return hipGetLastError();
 }
 #endif  // SFN_IN_PART(5)

@@ -98,10 +98,13 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
                        hipStream_t s);
 hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s);
 // Tuned multi-job pass: segments' kind = field * 2 + dir (+ 4 when lanes are
-// adjacent in memory), log_t in [6, 11]; kMultiFastThreads threads per workgroup.
+// adjacent in memory), log_t in [6, 11]; kMultiFastThreads threads per
+// workgroup.  bb_root: the BabyBear root table (2^11 Shoup pairs) when any
+// segment is BabyBear, else null; lanes_bytes: the lane-record region.
+// LDS = (bb_root ? 16 KiB : 0) + lanes_bytes + the largest tile.
 constexpr int kMultiFastThreads = 256;
 hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
-                             uint32_t xs, hipStream_t s);
+                             uint32_t xs, const void *bb_root, uint32_t lanes_bytes, hipStream_t s);
 hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);
 
 }  // namespace sfn

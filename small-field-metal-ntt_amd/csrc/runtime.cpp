@@ -738,7 +738,7 @@ namespace {
 // 32-point item per thread of a 256-thread workgroup.
 bool choose_lanes_multi(PassDesc &p, uint32_t batch) {
   if (p.log_t < 6 || p.log_t > (int)kMaxLogTile || p.logr != 5) return false;
-  uint32_t lw = 13 - (uint32_t)p.log_t;
+  uint32_t lw = std::min<uint32_t>(6, 13 - (uint32_t)p.log_t);  // lane records <= 2 KiB
   const uint64_t instances = (uint64_t)batch << (p.log_n - p.log_t);
   while (lw > 0 && (1ull << (lw - 1)) >= instances) --lw;  // no wholly idle lane groups
   if (p.lo > 0) {
@@ -789,6 +789,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
       tuned = choose_lanes_multi(q[j], m.batch) && (q[j].lane_contig || aligned) && fast_offsets_fit(q[j], lay, lay);
     }
     L.tuned = tuned;
+    size_t tile_bytes = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (k >= jp[j].size()) continue;
       const MultiJob &m = jobs[j];
@@ -808,7 +809,10 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
           d.kind += 4;
           if (p.log_w < 5) L.xs = std::max<uint32_t>(L.xs, std::min<uint32_t>(3, 5 - p.log_w));
         }
-        L.lds = std::max(L.lds, fast_lds_bytes(m.field, p.log_t, p.log_w));
+        if (m.field == SFN_FIELD_BABYBEAR) L.bb_root = d.a.tw;  // the context's root table, same for all
+        L.lanes_bytes = std::max<uint32_t>(L.lanes_bytes, (uint32_t)kLaneInfoBytes << p.log_w);
+        const size_t T = (size_t)1 << p.log_t;
+        tile_bytes = std::max(tile_bytes, ((T + T / 32) << p.log_w) * 4);
       } else {
         L.lds = std::max(L.lds, pass_lds_bytes(m.field, p.log_t, p.log_w));
       }
@@ -816,6 +820,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
       L.tiles += p.tiles;
       L.segs.push_back(d);
     }
+    if (tuned) L.lds = (L.bb_root ? ((size_t)8 << kMaxLogTile) : 0) + L.lanes_bytes + tile_bytes;
   }
   return SFN_OK;
 }
@@ -834,7 +839,8 @@ uint32_t multi_grid(const Context *ctx, const MultiLaunch &L) {
 }
 
 hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s) {
-  if (L.tuned) return launch_multi_fast(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, L.xs, s);
+  if (L.tuned)
+    return launch_multi_fast(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, L.xs, L.bb_root, L.lanes_bytes, s);
   return launch_multi(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, s);
 }
 

@@ -141,6 +141,8 @@ struct MultiLaunch {
   size_t lds = 0;
   bool tuned = false;  // k_multi_fast (radix-32 tiles) instead of the generic k_multi
   uint32_t xs = 0;     // XCD grouping of tiles (tuned launches)
+  const void *bb_root = nullptr;  // tuned launches with BabyBear segments: the root table
+  uint32_t lanes_bytes = 0;       // tuned launches: lane-record region
 };
 // Pass k of every job goes into launch k.  A launch uses the tuned kernel when
 // every one of its passes fits it (tile 2^6..2^11, offsets, alignment).
