@@ -766,11 +766,13 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
     const MultiJob &m = jobs[j];
     if (m.log_n == 0) continue;
     Schedule sc = ctx->schedule_for(m.field, m.log_n, m.batch, SFN_LAYOUT_SOA);
-    // many jobs fill the device together: prefer few, large passes unless tuned
-    if (ctx->tuned.find(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}) == ctx->tuned.end()) {
-      sc.max_log_t = kMaxLogTile;
-      sc.logr = 5;
-    }
+    // Many jobs fill the device together, so passes may be large: up to 2^11
+    // points, except that two 11-layer passes (three register rounds each)
+    // lose to three shorter ones (measured on 2^22 columns).  A tuned entry
+    // keeps its pass size; the radix is always the tuned kernel's.
+    if (ctx->tuned.find(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}) == ctx->tuned.end())
+      sc.max_log_t = m.log_n > 2 * (kMaxLogTile - 1) + 1 ? kMaxLogTile - 1 : kMaxLogTile;
+    sc.logr = 5;
     jp[j] = plan_ntt_passes(m.field, SFN_VARIANT_AUTO, m.dir, m.log_n, m.batch, SFN_LAYOUT_SOA, sc, false);
     nl = std::max(nl, jp[j].size());
   }
