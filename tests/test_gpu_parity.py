@@ -201,3 +201,24 @@ def test_determinism_and_device_tier(ctx):
                                  sfntt.LAYOUT_SOA, dout.ptr, dout.ptr, None))
     assert np.array_equal(dout.download(), x)
     assert ctx.twiddles_bytes() > 0
+
+
+@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+def test_large_single_transforms(ctx, field):
+    # Single columns beyond one pass pair (2^23..2^26): the four-step split in
+    # three passes.  2^23 against the oracle; 2^24..2^26 by the inverse round
+    # trip and, for BabyBear, out[0] = X_0 = sum of the inputs (mod p).
+    n = 1 << 23
+    x = data(field, n, 2323)
+    print(f"[parity] large field={field} log_n=23", flush=True)
+    y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.FORWARD, 23, 1, x)
+    assert_same(y, ref_fwd(field, x), f"forward 2^23 field={field}")
+    for log_n in (24, 26):
+        n = 1 << log_n
+        x = data(field, n, 1000 + log_n)
+        print(f"[parity] large field={field} log_n={log_n}", flush=True)
+        y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, 1, x)
+        z, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.INVERSE, log_n, 1, y)
+        assert_same(z, x, f"round trip 2^{log_n} field={field}")
+        if field == BB:
+            assert int(y[0]) == int(x.astype(np.uint64).sum() % orc.BB_P), f"X_0 at 2^{log_n}"
