@@ -1,0 +1,95 @@
+"""Writes the golden fixtures of the GPU parity tests from the CPU oracle.
+
+    python tests/golden/make_golden.py        (about a minute on 8 cores)
+
+fixtures.npz   small input/output vectors: BabyBear and M31 forward/inverse
+               at 2^4 and 2^9, coset LDEs (8,4,1) and (8,4,2) at the
+               reference shift and at shift 31 (src/lde/mod.rs:194-271 cases)
+digests.json   SHA-256 of the full outputs at the benchmark configurations,
+               over the little-endian u32 bytes in the stated layout:
+                 config1  BabyBear forward 2^16, seed 12345 + 2^16
+                 config2  M31 forward 2^20, seed 12345 + 2^20
+                 config3  coset LDE 2^20 x 256, 2x, seed 42 + 20 (inputs
+                          generated column by column): row-major with shift 31,
+                          and SoA with the reference shift
+                 config4  512 uneven columns (SplitMix64 seed 0x5EED0004:
+                          field = draw1 & 1, log_n = 10 + draw2 % 13, LCG seed
+                          = draw3), forward, outputs concatenated in order
+Inputs come from the oracle's restatement of the reference LCG; every output is
+the oracle's.  The files are data only (numpy arrays without pickling, JSON).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as orc  # noqa: E402
+
+M31, BB = orc.FIELD_M31, orc.FIELD_BB
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u4").tobytes()).hexdigest()
+
+
+def uneven_shapes(seed=0x5EED0004, count=512):
+    it = orc.splitmix64_stream(seed)
+    out = []
+    for _ in range(count):
+        a, b, c = next(it), next(it), next(it)
+        out.append((int(a & 1), 10 + int(b % 13), int(c)))
+    return out
+
+
+def small_fixtures() -> dict:
+    f = {}
+    for field, name in ((BB, "bb"), (M31, "m31")):
+        for log_n in (4, 9):
+            x = orc.lcg_fill(777 + log_n, 1 << log_n, field)
+            f[f"{name}_fwd_{log_n}_in"] = x
+            f[f"{name}_fwd_{log_n}_out"] = orc.m31_forward(x) if field == M31 else orc.bb_forward(x)
+            f[f"{name}_inv_{log_n}_out"] = orc.m31_inverse(x) if field == M31 else orc.bb_inverse(x)
+    for log_n, batch, a in ((8, 4, 1), (8, 4, 2)):
+        x = orc.lcg_fill(42, batch << log_n, BB)
+        f[f"lde_{log_n}_{batch}_{a}_in"] = x
+        f[f"lde_{log_n}_{batch}_{a}_ref_out"] = orc.coset_lde_batch(x, log_n, batch, a)
+        f[f"lde_{log_n}_{batch}_{a}_g31_out"] = orc.coset_lde_batch(x, log_n, batch, a, orc.bb_to_monty(31))
+    return f
+
+
+def digests() -> dict:
+    d = {}
+    x = orc.lcg_fill(42, 8 << 12, BB)
+    d["lde_12_8_1_ref"] = sha(orc.coset_lde_batch(x, 12, 8, 1))
+    d["lde_12_8_1_g31"] = sha(orc.coset_lde_batch(x, 12, 8, 1, orc.bb_to_monty(31)))
+    n = 1 << 16
+    d["config1_bb_fwd_2e16"] = sha(orc.bb_forward(orc.lcg_fill(12345 + n, n, BB)))
+    n = 1 << 20
+    d["config2_m31_fwd_2e20"] = sha(orc.m31_forward(orc.lcg_fill(12345 + n, n, M31)))
+    log_n, B, a = 20, 256, 1
+    x = orc.lcg_fill(42 + log_n, B << log_n, BB)  # column by column (SoA order)
+    y = orc.coset_lde_batch(x, log_n, B, a, 0)
+    d["config3_lde_2e20x256_2x_soa_refshift"] = sha(y)
+    y = orc.coset_lde_batch(x, log_n, B, a, orc.bb_to_monty(31))  # SoA result, shift 31
+    d["config3_lde_2e20x256_2x_rowmajor_shift31"] = sha(y.reshape(B, 2 << log_n).T)
+    h = hashlib.sha256()
+    for field, ln, seed in uneven_shapes():
+        x = orc.lcg_fill(seed, 1 << ln, field)
+        h.update(np.ascontiguousarray(orc.ntt_batch(field, False, ln, 1, x), dtype="<u4").tobytes())
+    d["config4_uneven_512_forward"] = h.hexdigest()
+    return d
+
+
+if __name__ == "__main__":
+    np.savez(os.path.join(HERE, "fixtures.npz"), **small_fixtures())
+    with open(os.path.join(HERE, "digests.json"), "w") as fh:
+        json.dump(digests(), fh, indent=1, sort_keys=True)
+        fh.write("\n")
+    print("wrote", os.listdir(HERE))
