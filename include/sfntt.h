@@ -106,6 +106,17 @@ sfn_status sfn_coset_lde_batch(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uin
                                uint32_t shift_raw, sfn_layout layout, const uint32_t *in,
                                uint32_t *out, uint64_t *device_ns);
 
+/* The same coset LDE with natural row order on both sides: input row i is the
+ * evaluation at g_n^i, output row k the evaluation at shift * g_N^k (the
+ * order of Plonky3's coset_lde_batch, which the reference converts to with
+ * bit_reverse_permutation in tests/ntt_cross_validation.rs:21-36, :64-66).
+ * No reference entry point does this; with SFN_LAYOUT_ROW_MAJOR and
+ * shift_raw = to_monty(31) it is the row-major matrix LDE of the benchmark the
+ * reference targets (benches/ntt_benchmark.rs:442). */
+sfn_status sfn_coset_lde_natural(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                                 uint32_t shift_raw, sfn_layout layout, const uint32_t *in,
+                                 uint32_t *out, uint64_t *device_ns);
+
 /* Element-wise product (NttBackend::pointwise_mul, src/ntt/mod.rs:46-48):
  * out[i] = a[i] * b[i] in the field; len may be any value >= 0. */
 sfn_status sfn_pointwise_mul(sfn_ctx *ctx, sfn_field field, const uint32_t *a, const uint32_t *b,
@@ -133,6 +144,10 @@ sfn_status sfn_ntt_device(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sf
 sfn_status sfn_coset_lde_batch_device(sfn_ctx *ctx, uint32_t log_n, uint32_t batch,
                                       uint32_t added_bits, uint32_t shift_raw, sfn_layout layout,
                                       const uint32_t *d_in, uint32_t *d_out, void *stream);
+
+sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t batch,
+                                        uint32_t added_bits, uint32_t shift_raw, sfn_layout layout,
+                                        const uint32_t *d_in, uint32_t *d_out, void *stream);
 
 /* Device time of the last launch sequence issued by this context on its own
  * stream (hipEvent pair), in nanoseconds; waits for it to finish. */

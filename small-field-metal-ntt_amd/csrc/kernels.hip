@@ -313,6 +313,49 @@ __global__ __launch_bounds__(256) void k_copy(const uint32_t *src, uint32_t *dst
     dst[i] = src[i];
 }
 
+// Row order of a 2^L-row array of `batch` columns between natural and
+// bit-reversed: dst row r = src row bitrev_L(r).  Consecutive threads walk
+// rows (SoA, rs == 1) or columns (row-major) so one side of the copy is
+// coalesced.  In place (src == dst) every pair r < bitrev(r) is swapped once.
+__device__ __forceinline__ void rows_index(uint64_t i, uint32_t L, uint32_t batch, uint64_t rs, uint64_t &r,
+                                           uint64_t &c) {
+  if (rs == 1) {
+    c = i >> L;
+    r = i & ((1ull << L) - 1);
+  } else {
+    r = i / batch;
+    c = i - r * batch;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rows_bitrev(const uint32_t *src, uint32_t *dst, uint32_t L, uint32_t batch,
+                                                     uint64_t scs, uint64_t srs, uint64_t dcs, uint64_t drs,
+                                                     uint64_t total) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r, c;
+    rows_index(i, L, batch, drs, r, c);
+    const uint64_t rr = L ? (uint64_t)(__brev((uint32_t)r) >> (32 - L)) : 0;
+    dst[c * dcs + r * drs] = src[c * scs + rr * srs];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rows_bitrev_inplace(uint32_t *data, uint32_t L, uint32_t batch, uint64_t cs,
+                                                             uint64_t rs, uint64_t total) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r, c;
+    rows_index(i, L, batch, rs, r, c);
+    const uint64_t rr = L ? (uint64_t)(__brev((uint32_t)r) >> (32 - L)) : 0;
+    if (r < rr) {
+      uint32_t *a = data + c * cs + r * rs, *b = data + c * cs + rr * rs;
+      const uint32_t t = *a;
+      *a = *b;
+      *b = t;
+    }
+  }
+}
+
 #endif  // SFN_IN_PART(4)
 
 // ------------------------------------------------------------ launchers
@@ -441,6 +484,20 @@ hipError_t launch_pointwise(int field, const uint32_t *a, const uint32_t *b, uin
   if (blocks > 65536) blocks = 65536;
   if (field == 0) hipLaunchKernelGGL((k_pointwise<M31>), dim3((uint32_t)blocks), dim3(256), 0, s, a, b, out, len);
   else hipLaunchKernelGGL((k_pointwise<BabyBear>), dim3((uint32_t)blocks), dim3(256), 0, s, a, b, out, len);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_bitrev(const uint32_t *src, uint32_t *dst, uint32_t L, uint32_t batch, uint64_t scs,
+                              uint64_t srs, uint64_t dcs, uint64_t drs, hipStream_t s) {
+  const uint64_t total = (uint64_t)batch << L;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (src == dst)
+    hipLaunchKernelGGL(k_rows_bitrev_inplace, dim3((uint32_t)blocks), dim3(256), 0, s, dst, L, batch, dcs, drs, total);
+  else
+    hipLaunchKernelGGL(k_rows_bitrev, dim3((uint32_t)blocks), dim3(256), 0, s, src, dst, L, batch, scs, srs, dcs, drs,
+                       total);
   return hipGetLastError();
 }
 

@@ -607,7 +607,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
 
 sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw,
                             int layout, const uint32_t *d_in, uint32_t *d_out, hipStream_t s,
-                            const Schedule *force) {
+                            const Schedule *force, bool natural) {
   const uint32_t LN = log_n + added_bits;
   const uint32_t shift_std =
       shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
@@ -627,6 +627,16 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   // K1: inverse layers [0, lo) (no normalisation) into the first n rows of out
   const uint32_t *mid_src = d_in;
   Layout mid_ls = Lin;
+  if (natural) {
+    // natural-order input: its rows, bit-reversed, become the first n rows of
+    // out, where the transform then runs in place
+    (void)timing_mark(s, true, "k_rows_bitrev<in>");
+    hipError_t e = launch_rows_bitrev(d_in, d_out, log_n, batch, Lin.cs, Lin.rs, Lout.cs, Lout.rs, s);
+    if (e != hipSuccess) return hip_fail(e, "row permutation launch");
+    (void)timing_mark(s, false, "");
+    mid_src = d_out;
+    mid_ls = Lout;
+  }
   for (size_t i = 0; i < outer.size(); ++i) {
     PassDesc p;
     p.lo = outer[i].first;
@@ -727,6 +737,12 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
     st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s, n_out, n_out);
     if (st) return st;
+  }
+  if (natural) {  // bit-reversed output rows -> natural order, in place
+    (void)timing_mark(s, true, "k_rows_bitrev<out>");
+    hipError_t e = launch_rows_bitrev(d_out, d_out, LN, batch, Lout.cs, Lout.rs, Lout.cs, Lout.rs, s);
+    if (e != hipSuccess) return hip_fail(e, "row permutation launch");
+    (void)timing_mark(s, false, "");
   }
   return SFN_OK;
 }
@@ -993,6 +1009,20 @@ sfn_status sfn_coset_lde_batch(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uin
   });
 }
 
+sfn_status sfn_coset_lde_natural(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                               uint32_t shift_raw, sfn_layout layout, const uint32_t *in, uint32_t *out,
+                               uint64_t *device_ns) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
+  if (st) return st;
+  if (!in || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  const size_t ni = (size_t)batch << log_n, no = (size_t)batch << (log_n + added_bits);
+  return host_call(c, in, ni, out, no, false, device_ns, [&](const uint32_t *di, uint32_t *dout) {
+    return c.run_lde(log_n, batch, added_bits, shift_raw, layout, di, dout, c.stream, nullptr, true);
+  });
+}
+
 sfn_status sfn_pointwise_mul(sfn_ctx *ctx, sfn_field field, const uint32_t *a, const uint32_t *b, uint32_t *out,
                              size_t len) {
   if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
@@ -1115,6 +1145,27 @@ sfn_status sfn_coset_lde_batch_device(sfn_ctx *ctx, uint32_t log_n, uint32_t bat
   const bool own = s == c.stream;
   if (own) (void)hipEventRecord(c.ev0, s);
   st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s);
+  if (own) {
+    (void)hipEventRecord(c.ev1, s);
+    c.timing_pending = st == SFN_OK;
+  }
+  return st;
+}
+
+sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                                        uint32_t shift_raw, sfn_layout layout, const uint32_t *d_in,
+                                        uint32_t *d_out, void *stream) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
+  if (st) return st;
+  if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
+  (void)hipSetDevice(c.device);
+  hipStream_t s = pick_stream(c, stream);
+  const bool own = s == c.stream;
+  if (own) (void)hipEventRecord(c.ev0, s);
+  st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s, nullptr, true);
   if (own) {
     (void)hipEventRecord(c.ev1, s);
     c.timing_pending = st == SFN_OK;

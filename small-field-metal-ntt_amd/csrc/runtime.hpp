@@ -115,8 +115,11 @@ class Context {
   // Plans and runs a transform on device buffers.
   sfn_status run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
                      const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr);
+  // natural: rows in natural order on both sides (input row i = P(g_n^i), output
+  // row k = P(shift * g_N^k)) instead of bit-reversed.
   sfn_status run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw, int layout,
-                     const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr);
+                     const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force = nullptr,
+                     bool natural = false);
 };
 
 std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,

@@ -111,9 +111,10 @@ class Context {
     return ns;
   }
   uint64_t coset_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw, Layout layout,
-                     const uint32_t *in, uint32_t *out) const {
+                     const uint32_t *in, uint32_t *out, bool natural = false) const {
     uint64_t ns = 0;
-    check(sfn_coset_lde_batch(ctx_, log_n, batch, added_bits, shift_raw, (sfn_layout)layout, in, out, &ns));
+    check((natural ? sfn_coset_lde_natural : sfn_coset_lde_batch)(ctx_, log_n, batch, added_bits, shift_raw,
+                                                                   (sfn_layout)layout, in, out, &ns));
     return ns;
   }
 
@@ -242,18 +243,29 @@ class CosetLdeBatch {
   // shift_raw = 0: the reference shift two_adic_generator(log_n + added_bits + 1).
   LdeResult execute(const std::vector<uint32_t> &data, uint32_t log_n, uint32_t batch, uint32_t added_bits,
                     uint32_t shift_raw = 0, Layout layout = Layout::SoA) const {
+    return run(data, log_n, batch, added_bits, shift_raw, layout, false);
+  }
+  // Natural row order on both sides (input row i = P(g_n^i), output row k =
+  // P(shift * g_N^k)), the order of Plonky3's coset_lde_batch.
+  LdeResult execute_natural(const std::vector<uint32_t> &data, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                            uint32_t shift_raw = 0, Layout layout = Layout::SoA) const {
+    return run(data, log_n, batch, added_bits, shift_raw, layout, true);
+  }
+  sfn_device_info device_info() const { return ctx_->device_info(); }
+
+ private:
+  LdeResult run(const std::vector<uint32_t> &data, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                uint32_t shift_raw, Layout layout, bool natural) const {
     if (data.size() != (size_t(batch) << log_n))  // src/lde/mod.rs:54-56
       throw InvalidSize(SFN_ERR_INVALID_SIZE, std::to_string(data.size()));
     if (log_n + added_bits + 1 > 27)  // two_adic_generator bound (src/field/babybear.rs:69-76)
       throw InvalidSize(SFN_ERR_INVALID_SIZE, "log_n + added_bits + 1 > 27");
     LdeResult r;
     r.data.resize(size_t(batch) << (log_n + added_bits));
-    r.total_ns = ctx_->coset_lde(log_n, batch, added_bits, shift_raw, layout, data.data(), r.data.data());
+    r.total_ns = ctx_->coset_lde(log_n, batch, added_bits, shift_raw, layout, data.data(), r.data.data(), natural);
     return r;
   }
-  sfn_device_info device_info() const { return ctx_->device_info(); }
 
- private:
   std::shared_ptr<Context> ctx_;
 };
 

@@ -68,6 +68,8 @@ SIGNATURES = {
                                ctypes.c_uint32, ctypes.c_int, u32p, u32p, u64p]),
     "sfn_coset_lde_batch": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.c_int, u32p, u32p, u64p]),
+    "sfn_coset_lde_natural": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_uint32, ctypes.c_int, u32p, u32p, u64p]),
     "sfn_pointwise_mul": (ctypes.c_int, [vp, ctypes.c_int, u32p, u32p, u32p, ctypes.c_size_t]),
     "sfn_buf_alloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
     "sfn_buf_free": (ctypes.c_int, [vp, vp]),
@@ -79,6 +81,9 @@ SIGNATURES = {
     "sfn_ntt_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_int, vp, vp, vp]),
     "sfn_coset_lde_batch_device": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                                  vp, vp, vp]),
+    "sfn_coset_lde_natural_device": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                                   vp, vp, vp]),
     "sfn_last_device_ns": (ctypes.c_int, [vp, u64p]),

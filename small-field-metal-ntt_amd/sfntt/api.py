@@ -131,13 +131,13 @@ class HipContext:
                                          _p(a), _p(out), ctypes.byref(ns)))
         return out, ns.value
 
-    def coset_lde(self, data, log_n, batch, added_bits, shift_raw=0, layout=L.LAYOUT_SOA):
+    def coset_lde(self, data, log_n, batch, added_bits, shift_raw=0, layout=L.LAYOUT_SOA, natural=False):
         a = _u32(data)
         out = np.empty(batch << (log_n + added_bits), dtype=np.uint32)
         ns = ctypes.c_uint64(0)
+        fn = self._lib.sfn_coset_lde_natural if natural else self._lib.sfn_coset_lde_batch
         with self.lock:
-            self.check(self._lib.sfn_coset_lde_batch(self.ptr, log_n, batch, added_bits, shift_raw,
-                                                     layout, _p(a), _p(out), ctypes.byref(ns)))
+            self.check(fn(self.ptr, log_n, batch, added_bits, shift_raw, layout, _p(a), _p(out), ctypes.byref(ns)))
         return out, ns.value
 
     def pointwise_mul(self, field, a, b):
@@ -352,6 +352,17 @@ class CosetLdeBatch:
         if a.size != (batch_size << log_n):  # src/lde/mod.rs:54-56
             raise InvalidSize(a.size)
         out, ns = self.ctx.coset_lde(a, log_n, batch_size, added_bits, shift_raw, layout)
+        return LdeResult(out, ns)
+
+    def execute_natural(self, data, log_n: int, batch_size: int, added_bits: int, shift_raw: int = 0,
+                        layout: int = L.LAYOUT_SOA) -> LdeResult:
+        """The same LDE with natural row order on both sides (input row i = P(g_n^i),
+        output row k = P(shift * g_N^k)): the order the reference converts to with
+        bit_reverse_permutation to compare against Plonky3 (tests/ntt_cross_validation.rs:21-36)."""
+        a = _u32(data)
+        if a.size != (batch_size << log_n):
+            raise InvalidSize(a.size)
+        out, ns = self.ctx.coset_lde(a, log_n, batch_size, added_bits, shift_raw, layout, natural=True)
         return LdeResult(out, ns)
 
     def device_info(self) -> dict:

@@ -75,6 +75,28 @@ def coset_lde_closed(col_raw: list[int], added_bits: int, shift_std: int | None)
     return out
 
 
+def coset_lde_natural_closed(col_raw: list[int], added_bits: int, shift_std: int | None) -> list[int]:
+    """Natural row order on both sides: input row k = P(w_n^k), output row k =
+    P(s * w_N^k) (Plonky3's coset_lde_batch order)."""
+    n = len(col_raw)
+    L = n.bit_length() - 1
+    rev_in = [0] * n
+    for k, v in enumerate(col_raw):
+        rev_in[bitrev(k, L)] = v
+    out_rev = coset_lde_closed(rev_in, added_bits, shift_std)
+    LN = L + added_bits
+    return [out_rev[bitrev(k, LN)] for k in range(len(out_rev))]
+
+
+def rows_bitrev(a, log_rows: int, batch: int, row_major: bool):
+    """Row r of the result is row bitrev(r) of `a` (numpy, SoA or row-major)."""
+    import numpy as np
+    idx = np.array([bitrev(r, log_rows) for r in range(1 << log_rows)], dtype=np.int64)
+    if row_major:
+        return a.reshape(1 << log_rows, batch)[idx].ravel()
+    return a.reshape(batch, 1 << log_rows)[:, idx].ravel()
+
+
 # ------------------------------------------------------------------ M31
 CIRCLE_GEN = (2, 1268011823)
 

@@ -132,6 +132,26 @@ int parity() {
       }
     }
   }
+  // natural row order on both sides: bit-reversed rows around the reference LDE
+  {
+    CosetLdeBatch lde(ctx);
+    const uint32_t log_n = 9, cols = 3, a = 2;
+    const size_t n = size_t(1) << log_n, N = n << a;
+    const std::vector<uint32_t> x = lcg(77, cols * n, true);
+    auto brev = [](size_t v, uint32_t bits) {
+      size_t r = 0;
+      for (uint32_t b = 0; b < bits; ++b) r |= ((v >> b) & 1) << (bits - 1 - b);
+      return r;
+    };
+    std::vector<uint32_t> xr(x.size()), y(cols * N), want(cols * N);
+    for (size_t c = 0; c < cols; ++c)
+      for (size_t r = 0; r < n; ++r) xr[c * n + r] = x[c * n + brev(r, log_n)];
+    orc_coset_lde_batch(xr.data(), log_n, cols, a, orc_bb_to_monty(31), 0, y.data(), 1);
+    for (size_t c = 0; c < cols; ++c)
+      for (size_t r = 0; r < N; ++r) want[c * N + r] = y[c * N + brev(r, log_n + a)];
+    LdeResult got = lde.execute_natural(x, log_n, cols, a, orc_bb_to_monty(31));
+    expect(got.data == want, "CosetLdeBatch.execute_natural 2^9 x 3, added_bits 2, shift 31");
+  }
   // pointwise_mul
   {
     Backend b = BbMetalR2(ctx);

@@ -222,3 +222,24 @@ def test_large_single_transforms(ctx, field):
         assert_same(z, x, f"round trip 2^{log_n} field={field}")
         if field == BB:
             assert int(y[0]) == int(x.astype(np.uint64).sum() % orc.BB_P), f"X_0 at 2^{log_n}"
+
+
+@pytest.mark.parametrize("log_n,batch,added_bits", [(1, 3, 1), (6, 5, 2), (12, 8, 1), (15, 24, 1), (20, 2, 1),
+                                                    (21, 2, 2)])
+@pytest.mark.parametrize("shift", [0, 31], ids=["ref_shift", "generator"])
+@pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
+def test_coset_lde_natural_order(ctx, log_n, batch, added_bits, shift, layout):
+    # natural rows on both sides = bit-reverse the input rows, reference LDE,
+    # bit-reverse the output rows (pinned against P(s * w_N^k) in test_oracle)
+    import closed_forms as cf
+    n = 1 << log_n
+    x = data(BB, n * batch, 4242 + log_n)
+    row_major = layout == sfntt.LAYOUT_ROW_MAJOR
+    shift_raw = 0 if shift == 0 else orc.bb_to_monty(shift)
+    print(f"[parity] lde natural log_n={log_n} batch={batch} a={added_bits} shift={shift} layout={layout}",
+          flush=True)
+    exp = orc.coset_lde_batch(cf.rows_bitrev(x, log_n, batch, row_major), log_n, batch, added_bits, shift_raw,
+                              layout=layout)
+    exp = cf.rows_bitrev(exp, log_n + added_bits, batch, row_major)
+    got = api.CosetLdeBatch(ctx=ctx).execute_natural(x, log_n, batch, added_bits, shift_raw, layout).data
+    assert_same(got, exp, f"natural LDE {log_n}x{batch} a={added_bits}")

@@ -209,6 +209,30 @@ def test_coset_lde_closed_form(orc, log_n, added_bits, shift):
         assert out[c * N:(c + 1) * N].tolist() == exp
 
 
+@pytest.mark.parametrize("log_n,added_bits", [(1, 1), (3, 1), (4, 2)])
+@pytest.mark.parametrize("shift", [None, 31])
+def test_natural_order_lde_is_rows_bitrev_around_the_reference(orc, log_n, added_bits, shift):
+    # The natural-order LDE (sfn_coset_lde_natural) is pinned as: bit-reverse
+    # the input rows, run the reference LDE, bit-reverse the output rows; and
+    # that composition is checked here against direct evaluation P(s * w_N^k).
+    batch = 2
+    n = 1 << log_n
+    data = orc.lcg_fill(4242, n * batch, orc.FIELD_BB)
+    shift_raw = 0 if shift is None else orc.bb_to_monty(shift)
+    N = n << added_bits
+    for row_major in (False, True):
+        x = data if not row_major else data.reshape(batch, n).T.copy().ravel()
+        lay = orc.LAYOUT_ROW_MAJOR if row_major else orc.LAYOUT_SOA
+        y = orc.coset_lde_batch(cf.rows_bitrev(x, log_n, batch, row_major), log_n, batch, added_bits, shift_raw,
+                                layout=lay)
+        y = cf.rows_bitrev(y, log_n + added_bits, batch, row_major)
+        if row_major:
+            y = y.reshape(N, batch).T.ravel()
+        for c in range(batch):
+            exp = cf.coset_lde_natural_closed(data[c * n:(c + 1) * n].tolist(), added_bits, shift)
+            assert y[c * N:(c + 1) * N].tolist() == exp
+
+
 def test_layouts_are_transposes(orc):
     log_n, batch, a = 6, 5, 1
     n, N = 1 << log_n, 1 << (log_n + a)
