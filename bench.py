@@ -31,6 +31,8 @@ COUNTERS_FILE = os.path.join(ROOT, "profiles", "lde_2e20x256_hbm_counters.txt")
 def kernel_bytes(label: str, log_n: int, batch: int, added_bits: int) -> int:
     """Algorithmic bytes of one launch: each value it must read once + each it writes once."""
     n, N = batch << log_n, batch << (log_n + added_bits)
+    if "rows_bitrev" in label:  # row-order permutation: every row read and written once
+        return 8 * (n if "<in>" in label else N)
     if "mid" in label.split("<")[0]:  # fused middle pass: n in, N out
         return 4 * (n + N)
     if "inv" in label:  # iDFT low layers: n in, n out
@@ -170,6 +172,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--added-bits", type=int, default=1)
     ap.add_argument("--semantics", choices=["as-stated", "reference"], default="as-stated")
+    ap.add_argument("--order", choices=["bitrev", "natural"], default="bitrev",
+                    help="row order of input and output (natural: sfn_coset_lde_natural, Plonky3's order)")
     ap.add_argument("--cpu-cols", type=int, default=24, help="columns in the CPU baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -222,9 +226,10 @@ def main() -> int:
     d_out = sfntt.DeviceBuffer(ctx, N * B)
     d_in.upload(x_dev)
 
+    lde_fn = lib.sfn_coset_lde_natural_device if args.order == "natural" else lib.sfn_coset_lde_batch_device
+
     def step():
-        ctx.check(lib.sfn_coset_lde_batch_device(ctx.ptr, log_n, B, a, shift_raw, layout,
-                                                 d_in.ptr, d_out.ptr, None))
+        ctx.check(lde_fn(ctx.ptr, log_n, B, a, shift_raw, layout, d_in.ptr, d_out.ptr, None))
 
     for _ in range(args.warmup):
         step()
@@ -263,17 +268,17 @@ def main() -> int:
 
     out_first = None
     cb = None
-    if args.cpu_cols > 0:
+    if args.cpu_cols > 0 and args.order == "bitrev":
         cols = min(args.cpu_cols, B)
         y = d_out.download()
         out_first = (y.reshape(N, B).T[:cols].ravel() if row_major else y[:cols * N])
         cb = cpu_baseline(x[:cols * n], log_n, cols, a, shift_raw, out_first)
     cb_all = None
-    if args.cpu_all_cols > 0:
+    if args.cpu_all_cols > 0 and args.order == "bitrev":
         cols = min(args.cpu_all_cols, B)
         cb_all = cpu_baseline_all_cores(x[:cols * n], log_n, cols, a, shift_raw)
     e2e = None
-    if args.e2e_steps > 0:
+    if args.e2e_steps > 0 and args.order == "bitrev":
         d_in.free()
         d_out.free()
         e2e = end_to_end(ctx, x_dev, log_n, B, a, shift_raw, layout, args.e2e_steps)
@@ -294,7 +299,7 @@ def main() -> int:
         "config": {
             "workload": f"babybear_coset_lde_2^{log_n}x{B}_{1 << a}x",
             "log_n": log_n, "batch": B, "added_bits": a,
-            "layout": "row_major" if row_major else "soa",
+            "layout": "row_major" if row_major else "soa", "row_order": args.order,
             "shift": "field generator 31" if row_major else "two_adic_generator(log_n+added_bits+1)",
             "device": info["name"] or info["gcn_arch"],
         },

@@ -46,6 +46,13 @@ __device__ __forceinline__ void bstore(uint32_t v, Rsrc r, uint32_t voff, uint32
   __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
 }
 
+template <int K>
+__host__ __device__ constexpr uint32_t brev_c(uint32_t i) {
+  uint32_t r = 0;
+  for (int b = 0; b < K; ++b) r |= ((i >> b) & 1u) << (K - 1 - b);
+  return r;
+}
+
 // v[i] *= tab[off + i * step] for the R values of an item (Montgomery table words).
 template <int R>
 __device__ __forceinline__ void mul_table(uint32_t (&v)[R], Rsrc tab, uint32_t off_bytes, uint32_t step_bytes) {
@@ -103,6 +110,9 @@ struct RoundIO {
   uint32_t sps, dps;        // elements between consecutive points
   uint32_t lo;
   uint32_t tps, tls;  // twist / post entry of (p, low) = (p << tps) + (low << tls)
+  // bit-reversed rows (lo == 0): point p of a lane sits bitrev_T(p) * srev
+  // elements after the lane's base (0: rows in storage order)
+  uint32_t srev, drev;
   uint32_t low0;      // low row bits of lane 0
   // LDS copies of this workgroup's twist / post tables, indexed by the point
   // (LDE middle pass with column-fastest lanes, which all share low0)
@@ -148,6 +158,13 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
 #pragma unroll
             for (int i = 0; i < R; ++i) v[i] = sp[(uint64_t)(i << B0) * io.sps];
           }
+        } else if (io.srev) {
+          // bitrev_T(pbase | i << B0) = bitrev_T(pbase) | bitrev_K(i) << (T - B0 - K)
+          const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
+          const uint32_t roff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pr * io.srev) * 4u;
+          const uint32_t rstep = io.srev * 4u;
+#pragma unroll
+          for (int i = 0; i < R; ++i) v[i] = bload(io.src, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
         } else {
           const uint32_t step = (io.sps << B0) * 4u;
 #pragma unroll
@@ -202,6 +219,12 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
 #pragma unroll
           for (int i = 0; i < R; ++i) dp[(uint64_t)(i << B0) * io.dps] = v[i];
         }
+      } else if (io.drev) {
+        const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
+        const uint32_t roff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pr * io.drev) * 4u;
+        const uint32_t rstep = io.drev * 4u;
+#pragma unroll
+        for (int i = 0; i < R; ++i) bstore(v[i], io.dst, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
       } else {
         const uint32_t step = (io.dps << B0) * 4u;
 #pragma unroll
@@ -274,6 +297,8 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.dps = (uint32_t)(((uint64_t)1 << a.lo) * a.dst_rs);
   io.lo = a.lo;
   io.low0 = l0.low;
+  io.srev = a.src_rev_bits ? (uint32_t)(a.src_rs << (a.src_rev_bits - LOG_T)) : 0u;
+  io.drev = a.dst_rev_bits ? (uint32_t)(a.dst_rs << (a.dst_rev_bits - LOG_T)) : 0u;
   io.ltw = io.lpost = nullptr;
   io.tps = a.tab_low_major ? 0u : a.lo;
   io.tls = a.tab_low_major ? (uint32_t)LOG_T : 0u;

@@ -48,6 +48,9 @@ struct PassArgs {
   const uint32_t *pointf_m;       // LDE: per-block point factors sigma_b^(p << lo) (blocks x 2^LOG_T)
   uint32_t xcd_share_log;         // 2^s consecutive tiles share 128-byte lines: keep them on one XCD
   uint32_t tab_low_major;         // twist_m / post_m entries ordered (low, p) instead of (p, low)
+  // rows stored in bit-reversed order over this many bits (0: storage order);
+  // tuned lane-contiguous passes with lo == 0 only
+  uint32_t src_rev_bits, dst_rev_bits;
 };
 
 // V1 (one launch per layer) arguments.

@@ -63,8 +63,11 @@ __device__ __forceinline__ void decode_lane(const PassArgs &a, int log_t, uint64
     c = (uint32_t)r;
   }
   const uint64_t row0 = (hb << hi) | low;
-  li.sbase = (uint64_t)c * a.src_cs + row0 * a.src_rs;
-  li.dbase = (uint64_t)c * a.dst_cs + row0 * a.dst_rs;
+  // logical row r of a bit-reversed array lives at physical row bitrev(r)
+  const uint64_t srow = a.src_rev_bits ? (uint64_t)(__brev((uint32_t)row0) >> (32 - a.src_rev_bits)) : row0;
+  const uint64_t drow = a.dst_rev_bits ? (uint64_t)(__brev((uint32_t)row0) >> (32 - a.dst_rev_bits)) : row0;
+  li.sbase = (uint64_t)c * a.src_cs + srow * a.src_rs;
+  li.dbase = (uint64_t)c * a.dst_cs + drow * a.dst_rs;
   li.hb = hb;
   li.low = low;
 }

@@ -52,10 +52,29 @@ Context::~Context() {
   rel(scratch_in);
   rel(scratch_out);
   rel(scratch_pp);
+  rel(scratch_lde);
+  if (scratch_ev) (void)hipEventDestroy(scratch_ev);
   for (auto e : tev) (void)hipEventDestroy(e);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+sfn_status Context::acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **out) {
+  if (!scratch_ev && hipEventCreateWithFlags(&scratch_ev, hipEventDisableTiming) != hipSuccess) {
+    scratch_ev = nullptr;
+    return fail(SFN_ERR_GPU_EXECUTION, "scratch event");
+  }
+  if (scratch_lde.bytes < bytes && scratch_ev_used) (void)hipEventSynchronize(scratch_ev);
+  sfn_status st = ensure(scratch_lde, bytes);
+  if (st) return st;
+  if (scratch_ev_used) (void)hipStreamWaitEvent(s, scratch_ev, 0);
+  *out = (uint32_t *)scratch_lde.ptr;
+  return SFN_OK;
+}
+
+void Context::release_lde_scratch(hipStream_t s) {
+  if (hipEventRecord(scratch_ev, s) == hipSuccess) scratch_ev_used = true;
 }
 
 sfn_status Context::ensure(DevBuf &b, size_t bytes) {
@@ -421,6 +440,8 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   a.order = p.order;
   a.lane_contig = p.lane_contig;
   a.instances = (uint64_t)batch << (p.log_n - p.log_t);
+  a.src_rev_bits = p.src_rev;
+  a.dst_rev_bits = p.dst_rev;
   if (field == SFN_FIELD_BABYBEAR) {
     const uint2 *f, *i;
     sfn_status st = ctx->bb_position(kMaxLogTile, &f, &i);
@@ -456,15 +477,16 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
 // lane 0: the point extent plus the lane spread must stay below 4 GiB.
 static bool fast_offsets_fit(const PassDesc &p, Layout ls, Layout ld) {
   const uint64_t T = 1ull << p.log_t, W = 1ull << p.log_w;
-  auto span = [&](Layout L) {
-    const uint64_t pts = (T << p.lo) * L.rs;
+  auto span = [&](Layout L, uint32_t rev) {
+    // bit-reversed rows: a lane's points spread over every row of the array
+    const uint64_t pts = rev ? (1ull << rev) * L.rs : (T << p.lo) * L.rs;
     uint64_t lane;
     if (p.order == kLowFastest) lane = W * L.rs;
     else if (p.order == kColFastest) lane = W * std::max(L.cs, L.rs);
     else lane = W * ((T << p.lo) * L.rs + L.cs);
     return (pts + lane + 1) * 4;
   };
-  return span(ls) < 0xFFFFFFFFull && span(ld) < 0xFFFFFFFFull;
+  return span(ls, p.src_rev) < 0xFFFFFFFFull && span(ld, p.dst_rev) < 0xFFFFFFFFull;
 }
 
 // Tuned-kernel fields of a pass's argument block (after make_pass_args):
@@ -540,6 +562,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   // the point-adjacent tuned kernel moves 16-byte vectors: needs aligned buffers
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
   if (p.fast && ctx->allow_fast && (p.lane_contig || aligned) && src_elems && dst_elems &&
+      (!(p.src_rev || p.dst_rev) || (p.lane_contig && p.lo == 0)) &&
       fast_offsets_fit(p, ls, ld)) {
     a.xcd_share_log = xcd_share_log(ctx, p);
     if ((st0 = fill_fast_args(ctx, field, dir, p, src_elems, dst_elems, &a))) return st0;
@@ -551,6 +574,8 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     (void)ctx->timing_mark(s, false, "");
     return SFN_OK;
   }
+  if (p.src_rev || p.dst_rev)  // bit-reversed rows are addressed by the tuned lane-contiguous kernel only
+    return ctx->fail(SFN_ERR_UNSUPPORTED, "bit-reversed row addressing needs the tuned kernel");
   if (p.fast) {  // fall back to the generic kernel with its own geometry
     PassDesc q = p;
     q.fast = false;
@@ -624,10 +649,34 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   const uint64_t n_in = (uint64_t)batch << log_n, n_out = (uint64_t)batch << LN;
   sfn_status st;
   auto outer = split_layers(lo, mlt);  // ascending (lo', log_t')
-  // K1: inverse layers [0, lo) (no normalisation) into the first n rows of out
+  // Natural row order.  Row-major columns with tuned outer passes: the first
+  // inverse pass reads the input rows bit-reversed and the last forward pass
+  // writes the output rows bit-reversed, so the permutation costs no traffic;
+  // the transform then runs in a scratch array, since a pass that moves rows
+  // cannot work in place.  Otherwise explicit row permutations of in and out.
+  uint32_t *work = d_out;
+  bool fused = false;
+  PassDesc f1, f3;
+  if (natural && layout == SFN_LAYOUT_ROW_MAJOR && allow_fast && sc.logr == 5 && !outer.empty() &&
+      outer[0].first == 0 && outer[0].second >= 6) {
+    f1.lo = f3.lo = 0;
+    f1.log_t = f3.log_t = (int)outer[0].second;
+    f1.logr = f3.logr = 5;
+    f1.log_n = log_n;
+    f3.log_n = LN;
+    if (choose_lanes_fast(f1, layout, batch, sc.log_w) && choose_lanes_fast(f3, layout, batch, sc.log_w)) {
+      f1.src_rev = log_n;
+      f3.dst_rev = LN;
+      if (f1.lane_contig && f3.lane_contig && fast_offsets_fit(f1, Lin, Lout) && fast_offsets_fit(f3, Lout, Lout)) {
+        if ((st = acquire_lde_scratch(n_out * 4, s, &work))) return st;
+        fused = true;
+      }
+    }
+  }
+  // K1: inverse layers [0, lo) (no normalisation) into the first n rows of work
   const uint32_t *mid_src = d_in;
   Layout mid_ls = Lin;
-  if (natural) {
+  if (natural && !fused) {
     // natural-order input: its rows, bit-reversed, become the first n rows of
     // out, where the transform then runs in place
     (void)timing_mark(s, true, "k_rows_bitrev<in>");
@@ -644,17 +693,18 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = log_n;
     if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
-    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, d_out, mid_ls, Lout, batch, log_n, s,
+    if (fused && i == 0) p = f1;
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 1, p, mid_src, work, mid_ls, Lout, batch, log_n, s,
                      mid_src == d_in ? n_in : n_out, n_out);
     if (st) return st;
-    mid_src = d_out;
+    mid_src = work;
     mid_ls = Lout;
   }
   // K2: inverse [lo, log_n) + n^-1 (shift*zeta_b)^j + forward [log_n-1, lo] per block
   {
     PassArgs a{};
     a.src = mid_src;
-    a.dst = d_out;
+    a.dst = work;
     a.src_cs = mid_ls.cs;
     a.src_rs = mid_ls.rs;
     a.dst_cs = Lout.cs;
@@ -735,10 +785,13 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     p.logr = (int)std::min<uint32_t>(sc.logr, outer[i].second);
     p.log_n = LN;
     if (!(p.logr == 5 && choose_lanes_fast(p, layout, batch, sc.log_w))) choose_lanes(p, layout, batch, sc.log_w);
-    st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, d_out, d_out, Lout, Lout, batch, LN, s, n_out, n_out);
+    if (fused && i == 0) p = f3;
+    st = launch_desc(this, SFN_FIELD_BABYBEAR, 0, p, work, fused && i == 0 ? d_out : work, Lout, Lout, batch, LN, s,
+                     n_out, n_out);
     if (st) return st;
   }
-  if (natural) {  // bit-reversed output rows -> natural order, in place
+  if (fused) release_lde_scratch(s);
+  if (natural && !fused) {  // bit-reversed output rows -> natural order, in place
     (void)timing_mark(s, true, "k_rows_bitrev<out>");
     hipError_t e = launch_rows_bitrev(d_out, d_out, LN, batch, Lout.cs, Lout.rs, Lout.cs, Lout.rs, s);
     if (e != hipSuccess) return hip_fail(e, "row permutation launch");

@@ -33,6 +33,7 @@ struct PassDesc {
   uint64_t tiles = 0;
   int src_sel = 0, dst_sel = 0;  // 0 = caller input, 1 = caller output, 2 = scratch
   bool scale = false;
+  uint32_t src_rev = 0, dst_rev = 0;  // rows bit-reversed over this many bits (tuned passes, lo == 0)
 };
 
 struct Schedule {
@@ -76,6 +77,11 @@ class Context {
 
   // scratch for the host-pointer API and V3 ping-pong
   DevBuf scratch_in, scratch_out, scratch_pp;
+  // intermediate array of the natural-order LDE; calls on any stream take
+  // turns on it through scratch_ev (recorded after each use)
+  DevBuf scratch_lde;
+  hipEvent_t scratch_ev = nullptr;
+  bool scratch_ev_used = false;
 
   std::map<TuneKey, Schedule> tuned;
   bool allow_fast = true;  // SFNTT_GENERIC_ONLY=1 at context creation turns the tuned kernels off
@@ -96,6 +102,9 @@ class Context {
   sfn_status hip_fail(hipError_t e, const char *what);
 
   sfn_status ensure(DevBuf &b, size_t bytes);
+  // scratch_lde of at least `bytes`, ordered after its previous user on any stream
+  sfn_status acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **out);
+  void release_lde_scratch(hipStream_t s);
   sfn_status upload_table(DevBuf &b, const void *host, size_t bytes);
   sfn_status bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);
