@@ -186,7 +186,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not offered "
-              "(the LDE shards by column with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(the LDE would shard by column, with no exchange between GPUs)", file=sys.stderr)
         return 2
 
     import numpy as np
