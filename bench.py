@@ -177,7 +177,7 @@ def main() -> int:
     ap.add_argument("--cpu-cols", type=int, default=24, help="columns in the CPU baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "uneven"], default="lde")
+    ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "uneven", "small"], default="lde")
     ap.add_argument("--e2e-steps", type=int, default=3,
                     help="timed host-pointer (PCIe-inclusive) LDE calls on pinned buffers (0 = skip)")
     ap.add_argument("--cpu-all-cols", type=int, default=128,

@@ -827,8 +827,24 @@ bool choose_lanes_multi(PassDesc &p, uint32_t batch) {
 
 }  // namespace
 
-sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vector<MultiLaunch> &out) {
+sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::vector<MultiLaunch> &out) {
   out.clear();
+  // Consecutive jobs of one shape whose columns follow each other in memory
+  // (input and output) are one batched job: their lanes then share tiles.
+  std::vector<MultiJob> jobs;
+  for (const MultiJob &m : jobs_in) {
+    if (!jobs.empty()) {
+      MultiJob &b = jobs.back();
+      const uint64_t span = (uint64_t)b.batch << b.log_n;
+      if (b.field == m.field && b.dir == m.dir && b.log_n == m.log_n && m.log_n > 0 &&
+          (uint64_t)b.batch + m.batch <= 0xFFFFFFFFull && m.src == b.src + span && m.dst == b.dst + span &&
+          ((b.src == b.dst) == (m.src == m.dst))) {
+        b.batch += m.batch;
+        continue;
+      }
+    }
+    jobs.push_back(m);
+  }
   std::vector<std::vector<PassDesc>> jp(jobs.size());
   size_t nl = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {

@@ -8,6 +8,9 @@ m31_20  M31 circle forward NTT, one column of 2^20, every variant, twiddles
 uneven  512 BabyBear/M31 columns with log_n ~ U{10..22} in one multi-job plan
         (one launch per pass, hipGraph replay) against launching every column
         on its own
+small   many small transforms (2^8 and 2^10 points, both fields; the batched
+        small-NTT regime of lattice schemes): one batched job, a plan of
+        single-column jobs (hipGraph), and single-column launches
 
 Device time is a hipEvent pair around the launch sequence with inputs
 resident in HBM; "call" time is the host-pointer entry point's wall clock
@@ -72,7 +75,56 @@ def run(name: str, ctx, steps: int, warmup: int) -> dict:
                            "log_n": log_n, "batch": 1}, "variants": r}
     if name == "uneven":
         return _uneven(ctx, steps, warmup)
+    if name == "small":
+        return _small(ctx, steps, warmup)
     raise ValueError(name)
+
+
+def _timed(ctx, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+    return (time.perf_counter() - t0) / steps
+
+
+def _small(ctx, steps, warmup):
+    lib = ctx._lib
+    out = {}
+    for field, log_n, cols in ((L.FIELD_BABYBEAR, 8, 1 << 16), (L.FIELD_M31, 8, 1 << 16),
+                               (L.FIELD_BABYBEAR, 10, 1 << 14), (L.FIELD_M31, 10, 1 << 14)):
+        n = 1 << log_n
+        total = cols * n
+        buf = sfntt.DeviceBuffer(ctx, total)
+        buf.upload(synth.field_data(field, 0x5A11 + log_n, total))
+        key = f"{'bb' if field else 'm31'}_2^{log_n}_x{cols}"
+        dt = _timed(ctx, lambda: ctx.check(lib.sfn_ntt_device(ctx.ptr, field, L.VARIANT_AUTO, L.FORWARD, log_n, cols,
+                                                              L.LAYOUT_SOA, buf.ptr, buf.ptr, None)), steps, warmup)
+        res = {"batch_job_us": round(dt * 1e6, 1), "batch_job_melem_s": round(total / dt / 1e6, 1)}
+        descs = [L.JobDesc(field, L.FORWARD, log_n, 1, ctypes.c_void_p(buf.ptr.value + 4 * n * c),
+                           ctypes.c_void_p(buf.ptr.value + 4 * n * c)) for c in range(cols)]
+        arr = (L.JobDesc * cols)(*descs)
+        plan = ctypes.c_void_p()
+        ctx.check(lib.sfn_plan_create(ctx.ptr, arr, cols, 1, ctypes.byref(plan)))
+        dt = _timed(ctx, lambda: ctx.check(lib.sfn_plan_launch(plan, None)), steps, warmup)
+        lib.sfn_plan_destroy(plan)
+        res.update({"plan_us": round(dt * 1e6, 1), "plan_melem_s": round(total / dt / 1e6, 1)})
+        few = 256  # single-column launches, timed on a subset and scaled per column
+        ptrs = [ctypes.c_void_p(buf.ptr.value + 4 * n * c) for c in range(few)]
+        dt = _timed(ctx, lambda: [ctx.check(lib.sfn_ntt_device(ctx.ptr, field, L.VARIANT_AUTO, L.FORWARD, log_n, 1,
+                                                                L.LAYOUT_SOA, p, p, None)) for p in ptrs],
+                    max(1, steps // 5), 1)
+        res.update({"per_column_launch_us": round(dt / few * 1e6, 2),
+                    "per_column_melem_s": round(few * n / dt / 1e6, 1)})
+        out[key] = res
+        buf.free()
+    best = max(v["plan_melem_s"] for v in out.values())
+    return {"metric": "batched small NTTs (2^8, 2^10 points), Melem/s", "value": best, "unit": "Melem/s",
+            "higher_is_better": True, "dtype": "u32", "data": "synthetic (reference LCG values)",
+            "config": {"workload": "small_ntt_batches"}, "modes": out}
 
 
 def _uneven(ctx, steps: int, warmup: int) -> dict:

@@ -69,6 +69,39 @@ def test_uneven_mixed_plan_matches_single_columns(ctx, use_graph):
         assert np.array_equal(y, fn(x)), (f, log_n, d)
 
 
+@pytest.mark.parametrize("in_place", [True, False], ids=["in_place", "out_of_place"])
+def test_plan_of_contiguous_small_columns(ctx, in_place):
+    # Many small single-column jobs laid out back to back in one buffer (the
+    # batched small-NTT case), with runs of changing shape and direction: the
+    # plan merges each run into one batched job; every column must still equal
+    # its own single-column transform.
+    runs = [(orc.FIELD_BB, 8, 300, 0), (orc.FIELD_M31, 8, 7, 0), (orc.FIELD_BB, 8, 50, 1), (orc.FIELD_BB, 10, 33, 0),
+            (orc.FIELD_M31, 10, 40, 1), (orc.FIELD_BB, 6, 5, 0)]
+    total = sum(c << ln for _, ln, c, _ in runs)
+    src = sfntt.DeviceBuffer(ctx, total)
+    dst = src if in_place else sfntt.DeviceBuffer(ctx, total)
+    xs, descs, off = [], [], 0
+    for f, ln, cols, d in runs:
+        for c in range(cols):
+            x = orc.lcg_fill(off + 17, 1 << ln, f)
+            xs.append((f, ln, d, off, x))
+            descs.append(L.JobDesc(f, d, ln, 1, ctypes.c_void_p(src.ptr.value + 4 * off),
+                                   ctypes.c_void_p(dst.ptr.value + 4 * off)))
+            off += 1 << ln
+    host = np.concatenate([x for *_, x in xs])
+    src.upload(host)
+    arr = (L.JobDesc * len(descs))(*descs)
+    plan = ctypes.c_void_p()
+    ctx.check(ctx._lib.sfn_plan_create(ctx.ptr, arr, len(descs), 1, ctypes.byref(plan)))
+    ctx.check(ctx._lib.sfn_plan_launch(plan, None))
+    ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+    ctx._lib.sfn_plan_destroy(plan)
+    y = dst.download()
+    for f, ln, d, o, x in xs:
+        exp = orc.ntt_batch(f, d == 1, ln, 1, x)
+        assert np.array_equal(y[o:o + (1 << ln)], exp), f"column at {o} field={f} log_n={ln} dir={d}"
+
+
 def test_graph_replays_each_match(ctx):
     # replay several times with host work between replays; every replay is a
     # fresh forward of the previous output, compared with the oracle each time
