@@ -47,7 +47,12 @@ typedef enum {
     SFN_ERR_NOT_READY = 10           /* server: job not finished yet */
 } sfn_status;
 
-typedef enum { SFN_FIELD_M31 = 0, SFN_FIELD_BABYBEAR = 1 } sfn_field;
+/* KoalaBear (p = 2^31 - 2^24 + 1) is on the reference's roadmap ("standard
+ * NTT", README.md:20) without code; it follows the BabyBear conventions
+ * exactly (Montgomery raw values, R = 2^32; forward = GS from natural order to
+ * bit-reversed evaluations; inverse scales by 1/n), with two-adicity 24 and the
+ * 2^24-th root 3^127 = 0x6ac49f88.  NTT only: the coset LDE is BabyBear's. */
+typedef enum { SFN_FIELD_M31 = 0, SFN_FIELD_BABYBEAR = 1, SFN_FIELD_KOALABEAR = 2 } sfn_field;
 
 /* The reference's four variants (src/ntt/mod.rs:19-25) plus SFN_VARIANT_AUTO,
  * which takes the tuned plan.  All variants return identical output. */
@@ -183,6 +188,8 @@ void sfn_plan_destroy(sfn_plan *plan);
 /* ---- autotuner: measures candidate schedules for (field, log_n, batch,
  * layout) on the device and stores the winner in the context's plan table.
  * The table can be saved/loaded as JSON text. ---- */
+/* field SFN_TUNE_LDE tunes the BabyBear coset LDE at 2x instead of an NTT. */
+#define SFN_TUNE_LDE 16
 sfn_status sfn_autotune(sfn_ctx *ctx, sfn_field field, uint32_t log_n, uint32_t batch,
                         sfn_layout layout, uint32_t *best_log_tile);
 sfn_status sfn_tuning_save(sfn_ctx *ctx, char *buf, size_t cap, size_t *len);

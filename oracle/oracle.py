@@ -15,7 +15,8 @@ import numpy as np
 
 M31_P = 0x7FFFFFFF
 BB_P = 0x78000001
-FIELD_M31, FIELD_BB = 0, 1
+KB_P = 0x7F000001
+FIELD_M31, FIELD_BB, FIELD_KB = 0, 1, 2
 LAYOUT_SOA, LAYOUT_ROW_MAJOR = 0, 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -48,6 +49,12 @@ def lib():
             ("orc_bb_sub", u32, [u32, u32]), ("orc_bb_mul", u32, [u32, u32]),
             ("orc_bb_reduce", u32, [u64]), ("orc_bb_inv", u32, [u32]),
             ("orc_bb_two_adic_generator", u32, [u32]),
+            ("orc_kb_monty_reduce", u32, [u64]), ("orc_kb_to_monty", u32, [u32]),
+            ("orc_kb_from_monty", u32, [u32]), ("orc_kb_add", u32, [u32, u32]),
+            ("orc_kb_sub", u32, [u32, u32]), ("orc_kb_mul", u32, [u32, u32]),
+            ("orc_kb_inv", u32, [u32]), ("orc_kb_two_adic_generator", u32, [u32]),
+            ("orc_kb_twiddles", i, [u32, i, _u32p]),
+            ("orc_kb_forward", i, [_u32p, sz]), ("orc_kb_inverse", i, [_u32p, sz]),
             ("orc_circle_odds_point", None, [u32, u64, _u32p, _u32p]),
             ("orc_bb_twiddles", i, [u32, i, _u32p]), ("orc_m31_twiddles", i, [u32, i, _u32p]),
             ("orc_m31_forward", i, [_u32p, sz]), ("orc_m31_inverse", i, [_u32p, sz]),
@@ -88,6 +95,14 @@ def bb_to_monty(v): return lib().orc_bb_to_monty(v)
 def bb_from_monty(v): return lib().orc_bb_from_monty(v)
 def bb_monty_reduce(t): return lib().orc_bb_monty_reduce(t)
 def bb_two_adic_generator(log_n): return lib().orc_bb_two_adic_generator(log_n)
+def kb_mul(a, b): return lib().orc_kb_mul(a, b)
+def kb_add(a, b): return lib().orc_kb_add(a, b)
+def kb_sub(a, b): return lib().orc_kb_sub(a, b)
+def kb_inv(a): return lib().orc_kb_inv(a)
+def kb_to_monty(v): return lib().orc_kb_to_monty(v)
+def kb_from_monty(v): return lib().orc_kb_from_monty(v)
+def kb_monty_reduce(t): return lib().orc_kb_monty_reduce(t)
+def kb_two_adic_generator(log_n): return lib().orc_kb_two_adic_generator(log_n)
 
 
 def circle_odds_point(log_size: int, index: int):
@@ -100,6 +115,12 @@ def circle_odds_point(log_size: int, index: int):
 def bb_twiddles(log_n: int, inverse: bool = False) -> np.ndarray:
     out = np.zeros(max((1 << log_n) - 1, 1), dtype=np.uint32)
     assert lib().orc_bb_twiddles(log_n, int(inverse), _ptr(out)) == 0
+    return out[: (1 << log_n) - 1]
+
+
+def kb_twiddles(log_n: int, inverse: bool = False) -> np.ndarray:
+    out = np.zeros(max((1 << log_n) - 1, 1), dtype=np.uint32)
+    assert lib().orc_kb_twiddles(log_n, int(inverse), _ptr(out)) == 0
     return out[: (1 << log_n) - 1]
 
 
@@ -125,6 +146,8 @@ def m31_forward(data): return _single(lib().orc_m31_forward, data)
 def m31_inverse(data): return _single(lib().orc_m31_inverse, data)
 def bb_forward(data): return _single(lib().orc_bb_forward, data)
 def bb_inverse(data): return _single(lib().orc_bb_inverse, data)
+def kb_forward(data): return _single(lib().orc_kb_forward, data)
+def kb_inverse(data): return _single(lib().orc_kb_inverse, data)
 
 
 def ntt_batch(field: int, inverse: bool, log_n: int, batch: int, data, layout: int = LAYOUT_SOA,
@@ -154,10 +177,10 @@ def coset_lde_batch(data, log_n: int, batch: int, added_bits: int, shift_raw: in
 
 # ------------------------------------------------------------ generators
 def lcg_fill(seed: int, n: int, field: int) -> np.ndarray:
-    """Reference LCG data: M31 canonical values or BabyBear Montgomery raw."""
+    """Reference LCG data: M31 canonical values, BabyBear or KoalaBear Montgomery raw."""
     out = np.empty(n, dtype=np.uint32)
-    p = M31_P if field == FIELD_M31 else BB_P
-    lib().orc_lcg_fill(seed, p, int(field == FIELD_BB), _ptr(out) if n else _u32p(), n)
+    p = {FIELD_M31: M31_P, FIELD_BB: BB_P, FIELD_KB: KB_P}[field]
+    lib().orc_lcg_fill(seed, p, field, _ptr(out) if n else _u32p(), n)
     return out
 
 

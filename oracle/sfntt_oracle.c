@@ -51,63 +51,91 @@ uint32_t orc_m31_inv(uint32_t a) {
     return r;
 }
 
-/* ------------------------------------------------------------- BabyBear */
+/* ------------------------------------------- Montgomery fields (R = 2^32) */
 
-#define BB_R2 1172168163u    /* R^2 mod p, src/field/babybear.rs:21 */
-#define BB_MONTY 2013265919u /* -p^-1 mod 2^32, src/field/babybear.rs:25 */
-#define BB_ROOT27 1476048622u /* Montgomery form of 440564289, babybear.rs:33 */
-#define BB_ONE 268435454u    /* R mod p, babybear.rs:18 */
+/* BabyBear and KoalaBear share one restatement of the reference's BabyBear
+ * field (src/field/babybear.rs); only the constants differ.  KoalaBear is on
+ * the reference's roadmap (README.md:20, "standard NTT") and has no code
+ * there: its constants are p = 2^31 - 2^24 + 1, the two-adicity 24 of p - 1 =
+ * 2^24 * 127, and the 2^24-th root 3^127 (3 generates F_p^*; 3^127 =
+ * 0x6ac49f88 is also Plonky3's KoalaBear two-adic generator). */
+typedef struct {
+    uint32_t p;
+    uint32_t monty;   /* -p^-1 mod 2^32 */
+    uint32_t r2;      /* R^2 mod p */
+    uint32_t one;     /* R mod p */
+    uint32_t root;    /* primitive 2^max_log-th root, Montgomery form */
+    uint32_t max_log; /* two-adicity of p - 1 */
+} mfield;
+
+static const mfield BB = {ORC_BB_P, 2013265919u /* babybear.rs:25 */, 1172168163u /* babybear.rs:21 */,
+                          268435454u /* babybear.rs:18 */, 1476048622u /* to_monty(440564289), babybear.rs:33 */,
+                          27};
+static const mfield KB = {ORC_KB_P, 0x7EFFFFFFu, 402124772u, 33554430u, 0 /* set by kb_field() */, 24};
+#define KB_ROOT24_STD 1791270792u /* 3^127 mod p */
 
 /* src/field/babybear.rs:40-51 */
-uint32_t orc_bb_monty_reduce(uint64_t t) {
-    uint32_t m = (uint32_t)t * BB_MONTY;
-    uint64_t mp = (uint64_t)m * ORC_BB_P;
+static uint32_t mf_reduce(const mfield *f, uint64_t t) {
+    uint32_t m = (uint32_t)t * f->monty;
+    uint64_t mp = (uint64_t)m * f->p;
     /* t + mp < 2^64 for every t = a*b with a, b < 2^32 used here */
     uint32_t r = (uint32_t)((t + mp) >> 32);
-    return r >= ORC_BB_P ? r - ORC_BB_P : r;
+    return r >= f->p ? r - f->p : r;
 }
-
-/* src/field/babybear.rs:54-59 */
-uint32_t orc_bb_to_monty(uint32_t v) { return orc_bb_monty_reduce((uint64_t)v * BB_R2); }
-
-/* src/field/babybear.rs:62-65 */
-uint32_t orc_bb_from_monty(uint32_t m) { return orc_bb_monty_reduce(m); }
-
-/* src/field/babybear.rs:113-122 */
-uint32_t orc_bb_add(uint32_t a, uint32_t b) {
+static uint32_t mf_to_monty(const mfield *f, uint32_t v) { return mf_reduce(f, (uint64_t)v * f->r2); } /* :54-59 */
+static uint32_t mf_add(const mfield *f, uint32_t a, uint32_t b) {                                     /* :113-122 */
     uint32_t s = a + b;
-    return s >= ORC_BB_P ? s - ORC_BB_P : s;
+    return s >= f->p ? s - f->p : s;
 }
-
-/* src/field/babybear.rs:124-131 */
-uint32_t orc_bb_sub(uint32_t a, uint32_t b) {
-    return a >= b ? a - b : ORC_BB_P - (b - a);
-}
-
-/* src/field/babybear.rs:133-137 */
-uint32_t orc_bb_mul(uint32_t a, uint32_t b) { return orc_bb_monty_reduce((uint64_t)a * b); }
-
-/* src/field/babybear.rs:139-145 */
-uint32_t orc_bb_reduce(uint64_t v) { return orc_bb_to_monty((uint32_t)(v % ORC_BB_P)); }
-
-/* src/field/babybear.rs:147-161 */
-uint32_t orc_bb_inv(uint32_t a) {
-    uint32_t r = BB_ONE, base = a, e = ORC_BB_P - 2;
+static uint32_t mf_sub(const mfield *f, uint32_t a, uint32_t b) { return a >= b ? a - b : f->p - (b - a); } /* :124-131 */
+static uint32_t mf_mul(const mfield *f, uint32_t a, uint32_t b) { return mf_reduce(f, (uint64_t)a * b); } /* :133-137 */
+static uint32_t mf_inv(const mfield *f, uint32_t a) {                                                   /* :147-161 */
+    uint32_t r = f->one, base = a, e = f->p - 2;
     while (e) {
-        if (e & 1) r = orc_bb_mul(r, base);
-        base = orc_bb_mul(base, base);
+        if (e & 1) r = mf_mul(f, r, base);
+        base = mf_mul(f, base, base);
         e >>= 1;
     }
     return r;
 }
-
-/* src/field/babybear.rs:69-76: square the 2^27-th root (27 - log_n) times. */
-uint32_t orc_bb_two_adic_generator(uint32_t log_n) {
-    if (log_n > 27) return 0;
-    uint32_t g = BB_ROOT27;
-    for (uint32_t i = 0; i < 27 - log_n; ++i) g = orc_bb_mul(g, g);
+/* src/field/babybear.rs:69-76: square the 2^max_log-th root (max_log - log_n) times. */
+static uint32_t mf_two_adic_generator(const mfield *f, uint32_t log_n) {
+    if (log_n > f->max_log) return 0;
+    uint32_t g = f->root;
+    for (uint32_t i = 0; i < f->max_log - log_n; ++i) g = mf_mul(f, g, g);
     return g;
 }
+
+static const mfield *kb_field(void) {
+    static mfield kb;
+    static int ready = 0;
+    if (!ready) {
+        kb = KB;
+        kb.root = mf_to_monty(&kb, KB_ROOT24_STD);
+        ready = 1;
+    }
+    return &kb;
+}
+static const mfield *mfield_of(int field) { return field == 2 ? kb_field() : &BB; }
+
+uint32_t orc_bb_monty_reduce(uint64_t t) { return mf_reduce(&BB, t); }
+uint32_t orc_bb_to_monty(uint32_t v) { return mf_to_monty(&BB, v); }
+uint32_t orc_bb_from_monty(uint32_t m) { return mf_reduce(&BB, m); } /* src/field/babybear.rs:62-65 */
+uint32_t orc_bb_add(uint32_t a, uint32_t b) { return mf_add(&BB, a, b); }
+uint32_t orc_bb_sub(uint32_t a, uint32_t b) { return mf_sub(&BB, a, b); }
+uint32_t orc_bb_mul(uint32_t a, uint32_t b) { return mf_mul(&BB, a, b); }
+uint32_t orc_bb_reduce(uint64_t v) { return mf_to_monty(&BB, (uint32_t)(v % ORC_BB_P)); } /* :139-145 */
+uint32_t orc_bb_inv(uint32_t a) { return mf_inv(&BB, a); }
+uint32_t orc_bb_two_adic_generator(uint32_t log_n) { return mf_two_adic_generator(&BB, log_n); }
+
+uint32_t orc_kb_monty_reduce(uint64_t t) { return mf_reduce(kb_field(), t); }
+uint32_t orc_kb_to_monty(uint32_t v) { return mf_to_monty(kb_field(), v); }
+uint32_t orc_kb_from_monty(uint32_t m) { return mf_reduce(kb_field(), m); }
+uint32_t orc_kb_add(uint32_t a, uint32_t b) { return mf_add(kb_field(), a, b); }
+uint32_t orc_kb_sub(uint32_t a, uint32_t b) { return mf_sub(kb_field(), a, b); }
+uint32_t orc_kb_mul(uint32_t a, uint32_t b) { return mf_mul(kb_field(), a, b); }
+uint32_t orc_kb_inv(uint32_t a) { return mf_inv(kb_field(), a); }
+uint32_t orc_kb_two_adic_generator(uint32_t log_n) { return mf_two_adic_generator(kb_field(), log_n); }
 
 /* ---------------------------------------------------------------- circle */
 
@@ -193,20 +221,22 @@ static uint64_t bitrev(uint64_t v, uint32_t bits) {
 
 /* src/field/babybear.rs:83-97 (forward) and src/ntt/babybear/twiddles.rs:13-19
  * (inverse = element-wise inv of the forward table). */
-int orc_bb_twiddles(uint32_t log_n, int inverse, uint32_t *out) {
-    if (log_n > 27) return -1;
+static int mf_twiddles(const mfield *f, uint32_t log_n, int inverse, uint32_t *out) {
+    if (log_n > f->max_log) return -1;
     for (uint32_t s = 0; s < log_n; ++s) {
         size_t half = (size_t)1 << s;
-        uint32_t g = orc_bb_two_adic_generator(s + 1);
-        uint32_t w = BB_ONE;
+        uint32_t g = mf_two_adic_generator(f, s + 1);
+        uint32_t w = f->one;
         uint32_t *layer = out + (half - 1);
         for (size_t j = 0; j < half; ++j) {
-            layer[j] = inverse ? orc_bb_inv(w) : w;
-            w = orc_bb_mul(w, g);
+            layer[j] = inverse ? mf_inv(f, w) : w;
+            w = mf_mul(f, w, g);
         }
     }
     return 0;
 }
+int orc_bb_twiddles(uint32_t log_n, int inverse, uint32_t *out) { return mf_twiddles(&BB, log_n, inverse, out); }
+int orc_kb_twiddles(uint32_t log_n, int inverse, uint32_t *out) { return mf_twiddles(kb_field(), log_n, inverse, out); }
 
 /* src/ntt/m31/twiddles.rs:19-43 (forward) and :47-68 (inverse): coset odds(log_n),
  * layer l takes x (or y on the last layer) of current.at(bitrev(i, log_size-1)),
@@ -302,7 +332,7 @@ int orc_m31_inverse(uint32_t *data, size_t n) {
 
 /* src/ntt/babybear/cpu_reference.rs:34-62 (forward_ntt_u32: GS-DIF, layers
  * log_n-1..0, position-indexed twiddle tw[layer][j]). */
-static void bb_forward_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
+static void mf_forward_tw(const mfield *f, uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (int layer = (int)log_n - 1; layer >= 0; --layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (half - 1);
@@ -310,56 +340,56 @@ static void bb_forward_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32
             for (size_t j = 0; j < half; ++j) {
                 size_t i0 = b * full + j, i1 = i0 + half;
                 uint32_t v0 = data[i0], v1 = data[i1];
-                data[i0] = orc_bb_add(v0, v1);
-                data[i1] = orc_bb_mul(orc_bb_sub(v0, v1), t[j]);
+                data[i0] = mf_add(f, v0, v1);
+                data[i1] = mf_mul(f, mf_sub(f, v0, v1), t[j]);
             }
         }
     }
 }
 
-int orc_bb_forward(uint32_t *data, size_t n) {
-    uint32_t log_n;
-    if (log2_exact(n, &log_n)) return -1;
-    if (n == 1) return 0;
-    if (log_n > 27) return -1;
-    uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_bb_twiddles(log_n, 0, tw);
-    bb_forward_tw(data, n, log_n, tw);
-    free(tw);
-    return 0;
-}
-
 /* src/ntt/babybear/cpu_reference.rs:69-103 (inverse_ntt_u32: CT-DIT, layers
  * 0..log_n-1, then multiply by inv(n)). */
-static void bb_inverse_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
+static void mf_inverse_tw(const mfield *f, uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
     for (uint32_t layer = 0; layer < log_n; ++layer) {
         size_t half = (size_t)1 << layer, full = half << 1;
         const uint32_t *t = tw + (half - 1);
         for (size_t b = 0; b < n / full; ++b) {
             for (size_t j = 0; j < half; ++j) {
                 size_t i0 = b * full + j, i1 = i0 + half;
-                uint32_t tmp = orc_bb_mul(data[i1], t[j]);
+                uint32_t tmp = mf_mul(f, data[i1], t[j]);
                 uint32_t v0 = data[i0];
-                data[i0] = orc_bb_add(v0, tmp);
-                data[i1] = orc_bb_sub(v0, tmp);
+                data[i0] = mf_add(f, v0, tmp);
+                data[i1] = mf_sub(f, v0, tmp);
             }
         }
     }
-    uint32_t inv_n = orc_bb_inv(orc_bb_reduce((uint64_t)n));
-    for (size_t i = 0; i < n; ++i) data[i] = orc_bb_mul(data[i], inv_n);
+    uint32_t inv_n = mf_inv(f, mf_to_monty(f, (uint32_t)(n % f->p)));
+    for (size_t i = 0; i < n; ++i) data[i] = mf_mul(f, data[i], inv_n);
 }
 
-int orc_bb_inverse(uint32_t *data, size_t n) {
+static void bb_forward_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
+    mf_forward_tw(&BB, data, n, log_n, tw);
+}
+static void bb_inverse_tw(uint32_t *data, size_t n, uint32_t log_n, const uint32_t *tw) {
+    mf_inverse_tw(&BB, data, n, log_n, tw);
+}
+
+static int mf_single(const mfield *f, int inverse, uint32_t *data, size_t n) {
     uint32_t log_n;
     if (log2_exact(n, &log_n)) return -1;
     if (n == 1) return 0;
-    if (log_n > 27) return -1;
+    if (log_n > f->max_log) return -1;
     uint32_t *tw = (uint32_t *)malloc((n - 1) * sizeof(uint32_t));
-    orc_bb_twiddles(log_n, 1, tw);
-    bb_inverse_tw(data, n, log_n, tw);
+    mf_twiddles(f, log_n, inverse, tw);
+    if (inverse) mf_inverse_tw(f, data, n, log_n, tw);
+    else mf_forward_tw(f, data, n, log_n, tw);
     free(tw);
     return 0;
 }
+int orc_bb_forward(uint32_t *data, size_t n) { return mf_single(&BB, 0, data, n); }
+int orc_bb_inverse(uint32_t *data, size_t n) { return mf_single(&BB, 1, data, n); }
+int orc_kb_forward(uint32_t *data, size_t n) { return mf_single(kb_field(), 0, data, n); }
+int orc_kb_inverse(uint32_t *data, size_t n) { return mf_single(kb_field(), 1, data, n); }
 
 /* ------------------------------------------------------- batched helpers */
 
@@ -379,8 +409,10 @@ static void *batch_worker(void *arg) {
         if (j->n == 1) continue;
         if (j->field == 0)
             j->dir == 0 ? m31_forward_tw(col, j->n, j->log_n, j->tw) : m31_inverse_tw(col, j->n, j->log_n, j->tw);
+        else if (j->dir == 0)
+            mf_forward_tw(mfield_of(j->field), col, j->n, j->log_n, j->tw);
         else
-            j->dir == 0 ? bb_forward_tw(col, j->n, j->log_n, j->tw) : bb_inverse_tw(col, j->n, j->log_n, j->tw);
+            mf_inverse_tw(mfield_of(j->field), col, j->n, j->log_n, j->tw);
     }
     return NULL;
 }
@@ -397,7 +429,7 @@ static void run_columns(int field, int dir, size_t n, uint32_t batch, uint32_t *
     uint32_t *tw = (uint32_t *)malloc((n > 1 ? n - 1 : 1) * sizeof(uint32_t));
     if (n > 1) {
         if (field == 0) orc_m31_twiddles(log_n, dir, tw);
-        else orc_bb_twiddles(log_n, dir, tw);
+        else mf_twiddles(mfield_of(field), log_n, dir, tw);
     }
     if (threads < 1) threads = 1;
     if ((uint32_t)threads > batch) threads = (int)batch;
@@ -429,7 +461,8 @@ static void run_columns(int field, int dir, size_t n, uint32_t batch, uint32_t *
  * single-column path at :438-480); the row-major layout is the transpose. */
 int orc_ntt_batch(int field, int dir, uint32_t log_n, uint32_t batch, int layout,
                   uint32_t *data, int threads) {
-    if ((field == 0 && log_n > 30) || (field == 1 && log_n > 27) || batch == 0) return -1;
+    if (field < 0 || field > 2 || batch == 0) return -1;
+    if ((field == 0 && log_n > 30) || (field != 0 && log_n > mfield_of(field)->max_log)) return -1;
     size_t n = (size_t)1 << log_n;
     if (layout == 0) {
         run_columns(field, dir, n, batch, data, threads);
@@ -479,7 +512,7 @@ int orc_coset_lde_batch(const uint32_t *in, uint32_t log_n, uint32_t batch,
      * in compute_shift_powers (src/lde/mod.rs:116-124). */
     uint32_t shift = shift_raw ? shift_raw : orc_bb_two_adic_generator(log_ext + 1);
     uint32_t *pw = (uint32_t *)malloc(n_ext * sizeof(uint32_t));
-    uint32_t cur = BB_ONE;
+    uint32_t cur = BB.one;
     for (size_t i = 0; i < n_ext; ++i) {
         pw[i] = cur;
         cur = orc_bb_mul(cur, shift);
@@ -541,7 +574,7 @@ uint64_t orc_lcg_fill(uint64_t seed, uint32_t modulus, int monty, uint32_t *out,
     for (size_t i = 0; i < n; ++i) {
         s = s * 6364136223846793005ull + 1ull;
         uint32_t v = (uint32_t)(s >> 33) % modulus;
-        out[i] = monty ? orc_bb_to_monty(v) : v;
+        out[i] = monty == 2 ? orc_kb_to_monty(v) : monty ? orc_bb_to_monty(v) : v;
     }
     return s;
 }

@@ -29,8 +29,8 @@ COMMON = [
 ]
 
 # (source, extra flags, object name)
-UNITS = [("kernels.hip", [f"-DSFN_PART={k}"], f"kernels_p{k}.o") for k in range(6)] + [
-    ("fast.hip", [f"-DSFN_PART={k}"], f"fast_p{k}.o") for k in range(6)] + [
+UNITS = [("kernels.hip", [f"-DSFN_PART={k}"], f"kernels_p{k}.o") for k in range(8)] + [
+    ("fast.hip", [f"-DSFN_PART={k}"], f"fast_p{k}.o") for k in range(8)] + [
     ("runtime.cpp", ["-x", "hip"], "runtime.o"),
     ("tables.cpp", ["-x", "hip"], "tables.o"),
     ("batch.cpp", ["-x", "hip"], "batch.o"),

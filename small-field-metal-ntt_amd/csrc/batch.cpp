@@ -132,7 +132,7 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
                         uint32_t *best_log_tile) {
   if (!cx) return SFN_ERR_INVALID_ARGUMENT;
   Context &c = cx->impl;
-  const bool lde = (int)field == 2;  // field 2 = the coset LDE at 2x
+  const bool lde = (int)field == SFN_TUNE_LDE;  // the coset LDE at 2x
   sfn_status st = lde ? sfn::validate_lde(&c, log_n, batch, 1, layout)
                       : sfn::validate_ntt(&c, field, SFN_VARIANT_AUTO, SFN_FORWARD, log_n, batch, layout);
   if (st) return st;
@@ -163,7 +163,7 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
         }
   }
   float best = 1e30f;
-  Schedule bs = c.schedule_for(lde ? 2 : field, log_n, batch, layout);
+  Schedule bs = c.schedule_for(lde ? SFN_TUNE_LDE : (int)field, log_n, batch, layout);
   const uint32_t *din = (const uint32_t *)c.scratch_in.ptr;
   uint32_t *dout = (uint32_t *)c.scratch_out.ptr;
   for (const Schedule &s : cands) {
@@ -188,7 +188,7 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
       bs = s;
     }
   }
-  c.tuned[sfn::TuneKey{lde ? 2 : (int)field, log_n, batch, (int)layout}] = bs;
+  c.tuned[sfn::TuneKey{lde ? SFN_TUNE_LDE : (int)field, log_n, batch, (int)layout}] = bs;
   if (best_log_tile) *best_log_tile = bs.max_log_t;
   c.last_error.clear();
   return SFN_OK;

@@ -54,17 +54,17 @@ __host__ __device__ constexpr uint32_t brev_c(uint32_t i) {
 }
 
 // v[i] *= tab[off + i * step] for the R values of an item (Montgomery table words).
-template <int R>
+template <class F, int R>
 __device__ __forceinline__ void mul_table(uint32_t (&v)[R], Rsrc tab, uint32_t off_bytes, uint32_t step_bytes) {
 #pragma unroll
-  for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], bload(tab, off_bytes, i * step_bytes));
+  for (int i = 0; i < R; ++i) v[i] = F::mul_mont(v[i], bload(tab, off_bytes, i * step_bytes));
 }
 
 // Same with the table staged in LDS, indexed by the point alone.
-template <int R, int B0>
+template <class F, int R, int B0>
 __device__ __forceinline__ void mul_lds_table(uint32_t (&v)[R], const uint32_t *tab, uint32_t pbase) {
 #pragma unroll
-  for (int i = 0; i < R; ++i) v[i] = BabyBear::mul_mont(v[i], tab[pbase | (i << B0)]);
+  for (int i = 0; i < R; ++i) v[i] = F::mul_mont(v[i], tab[pbase | (i << B0)]);
 }
 
 // Copies T words from global to LDS with 16-byte accesses (T >= 4 * blockDim
@@ -74,7 +74,7 @@ __device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *src, 
     *reinterpret_cast<uint4 *>(dst + e) = *reinterpret_cast<const uint4 *>(src + e);
 }
 
-// BabyBear twiddles from the LDS root table (Shoup pairs).
+// Montgomery-field twiddles from the LDS root table (Shoup pairs).
 template <int LOG_T, int DIR>
 struct RootTw {
   const uint2 *tab;
@@ -170,10 +170,10 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
 #pragma unroll
           for (int i = 0; i < R; ++i) v[i] = bload(io.src, voff, i * step);
         }
-        if constexpr (F::ID == 1 && DIR == 1) {
+        if constexpr (F::MONTY && DIR == 1) {
           if (io.has_twist) {
-            if constexpr (LT) mul_lds_table<R, B0>(v, io.ltw, pbase);
-            else mul_table<R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
+            if constexpr (LT) mul_lds_table<F, R, B0>(v, io.ltw, pbase);
+            else mul_table<F, R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
           }
         }
       } else {
@@ -184,27 +184,27 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
 #pragma unroll
       for (int i = 0; i < R; ++i) v[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
     }
-    if constexpr (F::ID == 0) tw.hb = li.hb;
+    if constexpr (!F::MONTY) tw.hb = li.hb;
     round_regs<F, DIR, K, B0>(v, pbase, tw);
     if constexpr (LAST) {
       if (!li.valid) continue;
-      if constexpr (F::ID == 1 && DIR == 0) {
+      if constexpr (F::MONTY && DIR == 0) {
         const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-        if (io.has_twist) mul_table<R>(v, io.twist, toff, 4u << (B0 + io.tps));
+        if (io.has_twist) mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
         if (io.has_post) {
           if constexpr (LT) {
             // keep the 32 table reads after the butterflies: hoisting them
             // above the round costs 32 live registers at the peak
             __builtin_amdgcn_sched_barrier(0);
-            mul_lds_table<R, B0>(v, io.lpost, pbase);
+            mul_lds_table<F, R, B0>(v, io.lpost, pbase);
           }
-          else mul_table<R>(v, io.post, toff, 4u << (B0 + io.tps));
+          else mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
         }
       }
       if (io.has_scale) {
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          if constexpr (F::ID == 1) v[i] = BabyBear::mul_tw(v[i], make_uint2(io.scale_w, io.scale_q));
+          if constexpr (F::MONTY) v[i] = F::mul_tw(v[i], make_uint2(io.scale_w, io.scale_q));
           else v[i] = M31::mul(v[i], io.scale_w);
         }
       }
@@ -250,16 +250,16 @@ __device__ __forceinline__ void rounds(uint32_t *tile, uint32_t log_w, const Lan
   }
 }
 
-// LDS: [LaneInfo x W][root table T x uint2 (BabyBear)][tile W x (T + T/32)]
+// LDS: [LaneInfo x W][root table T x uint2 (Montgomery fields)][tile W x (T + T/32)]
 __host__ __device__ constexpr size_t lds_bytes(int field, int log_t, uint32_t log_w) {
-  return (kLaneInfoBytes << log_w) + (field == 1 ? ((size_t)8 << log_t) : 0) +
+  return (kLaneInfoBytes << log_w) + (field != 0 ? ((size_t)8 << log_t) : 0) +
          ((((size_t)1 << log_t) + ((size_t)1 << log_t) / 32) << log_w) * 4;
 }
 
 template <class F, int DIR, int LOG_T>
 __device__ __forceinline__ typename TwSel<F, DIR, LOG_T>::type make_tw(const PassArgs &a, unsigned char *tabp) {
   typename TwSel<F, DIR, LOG_T>::type tw;
-  if constexpr (F::ID == 1) {
+  if constexpr (F::MONTY) {
     tw.tab = reinterpret_cast<const uint2 *>(tabp);
   } else {
     tw.tbl = reinterpret_cast<const uint32_t *>(a.tw);
@@ -271,7 +271,7 @@ __device__ __forceinline__ typename TwSel<F, DIR, LOG_T>::type make_tw(const Pas
   return tw;
 }
 
-// a.tw (BabyBear) = root table g_{2^11}^e, e < 2^11 (Shoup pairs)
+// a.tw (Montgomery fields) = root table g_{2^11}^e, e < 2^11 (Shoup pairs)
 template <int LOG_T>
 __device__ __forceinline__ void load_root(const PassArgs &a, uint2 *tab) {
   const uint2 *root = reinterpret_cast<const uint2 *>(a.tw);
@@ -311,7 +311,7 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
 }
 
 // One tile (W lanes) of a tuned pass; any blockDim (threads loop over items).
-// BabyBear twiddles come from an LDS root table of 2^ROOT_LOG entries
+// Montgomery-field twiddles come from an LDS root table of 2^ROOT_LOG entries
 // (g_{2^ROOT_LOG}^e): the twiddle of layer t, position j is entry
 // j << (ROOT_LOG - t - 1) whatever the sub-transform size, so one table of
 // 2^11 entries serves every tile size.  The caller fills it before the call
@@ -323,7 +323,7 @@ __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, L
   RoundIO io;
   setup<LOG_T>(a, lanes, io, tile_id);
   __syncthreads();
-  if constexpr (F::ID == 1) {
+  if constexpr (F::MONTY) {
     RootTw<ROOT_LOG, DIR> tw;
     tw.tab = root;
     rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, a.log_w, lanes, tw, io);
@@ -339,8 +339,8 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   uint2 *root = reinterpret_cast<uint2 *>(smem + (kLaneInfoBytes << a.log_w));
   uint32_t *tile = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(root) +
-                                                (F::ID == 1 ? ((size_t)8 << LOG_T) : 0));
-  if constexpr (F::ID == 1) load_root<LOG_T>(a, root);
+                                                (F::MONTY ? ((size_t)8 << LOG_T) : 0));
+  if constexpr (F::MONTY) load_root<LOG_T>(a, root);
   fast_tile<F, DIR, LOG_T, CONTIG, LOG_T>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
 }
 
@@ -419,10 +419,10 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
 #pragma unroll
     for (int i = 0; i < R; ++i) v[i] = c[i];
     if constexpr (LT) {
-      mul_lds_table<R, 0>(v, tabs + pbase * R, 0);
+      mul_lds_table<F, R, 0>(v, tabs + pbase * R, 0);
     } else {
       const Rsrc pf_r = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
-      mul_table<R>(v, pf_r, pbase * (4u * R), 4u);
+      mul_table<F, R>(v, pf_r, pbase * (4u * R), 4u);
       io.post = make_rsrc(a.post_m + b * n, n * 4);
     }
     round_regs<F, 0, K, B0>(v, pbase, twf);
@@ -490,12 +490,20 @@ hipError_t launch_fast_bb_fwd(int lt, bool c, const PassArgs &a, uint64_t t, hip
 #if SFN_IN_PART(3)
 hipError_t launch_fast_bb_inv(int lt, bool c, const PassArgs &a, uint64_t t, hipStream_t s) { return fast::launch_fd<BabyBear, 1>(lt, c, a, t, s); }
 #endif
+#if SFN_IN_PART(6)
+hipError_t launch_fast_kb_fwd(int lt, bool c, const PassArgs &a, uint64_t t, hipStream_t s) { return fast::launch_fd<KoalaBear, 0>(lt, c, a, t, s); }
+#endif
+#if SFN_IN_PART(7)
+hipError_t launch_fast_kb_inv(int lt, bool c, const PassArgs &a, uint64_t t, hipStream_t s) { return fast::launch_fd<KoalaBear, 1>(lt, c, a, t, s); }
+#endif
 
 #if SFN_IN_PART(4)
 hipError_t launch_fast_m31_fwd(int, bool, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_fast_m31_inv(int, bool, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_fast_bb_fwd(int, bool, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_fast_bb_inv(int, bool, const PassArgs &, uint64_t, hipStream_t);
+hipError_t launch_fast_kb_fwd(int, bool, const PassArgs &, uint64_t, hipStream_t);
+hipError_t launch_fast_kb_inv(int, bool, const PassArgs &, uint64_t, hipStream_t);
 
 size_t fast_lds_bytes(int field, int log_t, uint32_t log_w) { return fast::lds_bytes(field, log_t, log_w); }
 
@@ -504,6 +512,8 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
   if (tiles == 0) return hipSuccess;
   if (field == 0) return dir == 0 ? launch_fast_m31_fwd(log_t, contig, a, tiles, s)
                                   : launch_fast_m31_inv(log_t, contig, a, tiles, s);
+  if (field == 2) return dir == 0 ? launch_fast_kb_fwd(log_t, contig, a, tiles, s)
+                                  : launch_fast_kb_inv(log_t, contig, a, tiles, s);
   return dir == 0 ? launch_fast_bb_fwd(log_t, contig, a, tiles, s) : launch_fast_bb_inv(log_t, contig, a, tiles, s);
 }
 
@@ -567,8 +577,8 @@ __device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, ui
   }
 }
 
-// LDS: [BabyBear root table, 2^11 Shoup pairs, loaded once][lane records:
-// lanes_bytes][tile]
+// LDS: [root table of the launch's Montgomery field (BabyBear or KoalaBear;
+// one per launch), 2^11 Shoup pairs, loaded once][lane records: lanes_bytes][tile]
 __global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc *segs, uint32_t nseg,
                                                                   uint64_t total_tiles, uint32_t xs,
                                                                   const uint2 *root_g, uint32_t lanes_bytes) {
@@ -596,15 +606,19 @@ __global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc 
     const PassArgs a = segs[lo].a;
     const uint32_t local = (uint32_t)(t - segs[lo].tile_begin);
     const uint32_t log_t = segs[lo].log_t;
-    switch (segs[lo].kind) {  // field * 2 + dir, + 4 when lanes are adjacent in memory
+    switch (segs[lo].kind) {  // field * 2 + dir, + 6 when lanes are adjacent in memory
       case 0: multi_tile<M31, 0, false>(log_t, a, local, lanes, root, tile_lds); break;
       case 1: multi_tile<M31, 1, false>(log_t, a, local, lanes, root, tile_lds); break;
       case 2: multi_tile<BabyBear, 0, false>(log_t, a, local, lanes, root, tile_lds); break;
       case 3: multi_tile<BabyBear, 1, false>(log_t, a, local, lanes, root, tile_lds); break;
-      case 4: multi_tile<M31, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
-      case 5: multi_tile<M31, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
-      case 6: multi_tile<BabyBear, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
-      case 7: multi_tile<BabyBear, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 4: multi_tile<KoalaBear, 0, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 5: multi_tile<KoalaBear, 1, false>(log_t, a, local, lanes, root, tile_lds); break;
+      case 6: multi_tile<M31, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 7: multi_tile<M31, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 8: multi_tile<BabyBear, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 9: multi_tile<BabyBear, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 10: multi_tile<KoalaBear, 0, true>(log_t, a, local, lanes, root, tile_lds); break;
+      case 11: multi_tile<KoalaBear, 1, true>(log_t, a, local, lanes, root, tile_lds); break;
       default: break;
     }
     __syncthreads();  // the next tile reuses the LDS

@@ -2,13 +2,16 @@
 //
 // M31 (p = 2^31 - 1): canonical u32 values, Mersenne folding
 //   (src/field/m31.rs:101-115, shaders/fields/m31_field.metal:18-37).
-// BabyBear (p = 2^31 - 2^27 + 1): raw Montgomery values (R = 2^32)
-//   (src/field/babybear.rs:40-51, shaders/fields/babybear_field.metal:23-43).
+// Montgomery fields, R = 2^32, raw Montgomery values:
+//   BabyBear  p = 2^31 - 2^27 + 1 (src/field/babybear.rs:40-51,
+//             shaders/fields/babybear_field.metal:23-43);
+//   KoalaBear p = 2^31 - 2^24 + 1 (same arithmetic, two-adicity 24; not in
+//             the reference, which lists it as future work in README.md:20).
 //
 // Twiddle multiplications use a precomputed companion (Shoup, "NTL: A Library
 // for doing Number Theory", MulModPrecon): for a constant w in [0, p) and
 // wq = floor(w * 2^32 / p), q = mulhi(x, wq) and r = x*w - q*p (mod 2^32)
-// lies in [0, 2p) for every 32-bit x.  For BabyBear the raw Montgomery value
+// lies in [0, 2p) for every 32-bit x.  For a Montgomery field the raw value
 // x = X*R is multiplied by the *standard* value of the twiddle, which gives
 // the raw Montgomery form of X*W directly.
 #pragma once
@@ -26,6 +29,7 @@ __device__ __forceinline__ uint32_t canon(uint32_t a) {
 struct M31 {
   static constexpr uint32_t P = 0x7FFFFFFFu;
   static constexpr int ID = 0;
+  static constexpr bool MONTY = false;
   using Tw = uint32_t;  // twiddles are plain canonical values
 
   __device__ static __forceinline__ uint32_t add(uint32_t a, uint32_t b) { return canon<P>(a + b); }
@@ -40,17 +44,21 @@ struct M31 {
   __device__ static __forceinline__ uint32_t mul_tw(uint32_t a, Tw w) { return mul(a, w); }
 };
 
-struct BabyBear {
-  static constexpr uint32_t P = 0x78000001u;
-  static constexpr uint32_t PINV_NEG = 0x77FFFFFFu;  // -p^-1 mod 2^32
-  static constexpr int ID = 1;
+// PINV = p^-1 mod 2^32; both moduli are below 2^31, so sums of two canonical
+// values and the Shoup remainders fit in 32 bits.
+template <uint32_t P_, uint32_t PINV_, int ID_>
+struct MontyField {
+  static constexpr uint32_t P = P_;
+  static constexpr uint32_t PINV = PINV_;
+  static constexpr uint32_t PINV_NEG = 0u - PINV_;  // -p^-1 mod 2^32
+  static constexpr int ID = ID_;
+  static constexpr bool MONTY = true;
   using Tw = uint2;  // .x = standard value w, .y = floor(w * 2^32 / p)
+  static_assert((uint32_t)(P_ * PINV_) == 1u, "PINV is p^-1 mod 2^32");
 
   __device__ static __forceinline__ uint32_t add(uint32_t a, uint32_t b) { return canon<P>(a + b); }
   __device__ static __forceinline__ uint32_t sub(uint32_t a, uint32_t b) { return canon<P>(a + P - b); }
-  // q * p mod 2^32 with p = 2^31 - 2^27 + 1, using shifts instead of a
-  // quarter-rate 32-bit multiply.
-  __device__ static __forceinline__ uint32_t mulp(uint32_t q) { return q + (q << 31) - (q << 27); }
+  __device__ static __forceinline__ uint32_t mulp(uint32_t q) { return q * P; }
   // x any 32-bit value; result canonical.
   __device__ static __forceinline__ uint32_t mul_tw(uint32_t x, Tw w) {
     uint32_t q = __umulhi(x, w.y);
@@ -67,7 +75,7 @@ struct BabyBear {
   // 2^32 with no borrow, so (t - m*p) / 2^32 = t_hi - mulhi(m, p) in (-p, p).
   __device__ static __forceinline__ uint32_t mul_mont(uint32_t x, uint32_t wm) {
     const uint64_t t = (uint64_t)x * wm;
-    const uint32_t m = (uint32_t)t * 0x88000001u;  // p^-1 mod 2^32
+    const uint32_t m = (uint32_t)t * PINV;
     const uint32_t r = (uint32_t)(t >> 32) - __umulhi(m, P);
     return min(r, r + P);
   }
@@ -79,6 +87,9 @@ struct BabyBear {
     return canon<P>((uint32_t)(s >> 32));
   }
 };
+
+using BabyBear = MontyField<0x78000001u, 0x88000001u, 1>;
+using KoalaBear = MontyField<0x7F000001u, 0x81000001u, 2>;
 
 // ---- butterflies ----
 // CT (decimation in time):  (x, y) -> (x + w*y, x - w*y)
@@ -92,21 +103,11 @@ __device__ __forceinline__ void bfly_ct(uint32_t &x, uint32_t &y, typename F::Tw
 }
 
 template <class F>
-__device__ __forceinline__ void bfly_gs(uint32_t &x, uint32_t &y, typename F::Tw w);
-
-template <>
-__device__ __forceinline__ void bfly_gs<M31>(uint32_t &x, uint32_t &y, M31::Tw w) {
+__device__ __forceinline__ void bfly_gs(uint32_t &x, uint32_t &y, typename F::Tw w) {
   uint32_t a = x, b = y;
-  x = M31::add(a, b);
-  y = M31::mul(M31::sub(a, b), w);
-}
-
-template <>
-__device__ __forceinline__ void bfly_gs<BabyBear>(uint32_t &x, uint32_t &y, BabyBear::Tw w) {
-  uint32_t a = x, b = y;
-  x = BabyBear::add(a, b);
-  // a + p - b is in (0, 2p): Shoup accepts any 32-bit input.
-  y = BabyBear::mul_tw(a + BabyBear::P - b, w);
+  x = F::add(a, b);
+  if constexpr (F::MONTY) y = F::mul_tw(a + F::P - b, w);  // (0, 2p): Shoup accepts any 32-bit input
+  else y = F::mul(F::sub(a, b), w);
 }
 
 // Either butterfly with twiddle 1: (x, y) -> (x + y, x - y).

@@ -1,5 +1,7 @@
-// Hand-written gfx950 kernels for the M31 circle NTT, the BabyBear NTT and the
-// BabyBear coset LDE.
+// Hand-written gfx950 kernels for the M31 circle NTT, the BabyBear and
+// KoalaBear NTTs and the BabyBear coset LDE.  KoalaBear (p = 2^31 - 2^24 + 1)
+// shares every BabyBear kernel through the MontyField template: same
+// Montgomery form, same layer order and twiddle structure, its own constants.
 //
 // Pass kernel: one workgroup owns 2^log_w independent sub-transforms ("lanes")
 // of T = 2^LOG_T points each, i.e. layers [lo, lo + LOG_T) of every column.
@@ -28,9 +30,9 @@
 
 namespace sfn {
 
-// The pass instantiations are split over translation units (SFN_PART 0-3,
-// one per field x direction; 4 = everything else) so the build compiles them
-// in parallel.  Without SFN_PART one unit holds everything.
+// The pass instantiations are split over translation units (SFN_PART 0-3
+// and 6-7, one per field x direction; 4 = everything else, 5 = the
+// many-job launch) so the build compiles them in parallel.  Without SFN_PART one unit holds everything.
 #ifndef SFN_PART
 #define SFN_PART -1
 #endif
@@ -59,7 +61,7 @@ __device__ __forceinline__ void lds_round(uint32_t *tile, uint32_t log_w, const 
       const int p = pbase | (i << B0);
       v[i] = lt[Tile<LOG_T>::idx(p)];
     }
-    if constexpr (F::ID == 0) tw.hb = lanes[lane].hb;
+    if constexpr (!F::MONTY) tw.hb = lanes[lane].hb;
     round_regs<F, DIR, K, B0>(v, pbase, tw);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -87,6 +89,10 @@ template <>
 __device__ __forceinline__ uint32_t scale_mul<BabyBear>(uint32_t v, uint32_t w, uint32_t q) {
   return BabyBear::mul_tw(v, make_uint2(w, q));
 }
+template <>
+__device__ __forceinline__ uint32_t scale_mul<KoalaBear>(uint32_t v, uint32_t w, uint32_t q) {
+  return KoalaBear::mul_tw(v, make_uint2(w, q));
+}
 
 // Dynamic LDS layout: [LaneInfo x W][BB small table, T uint2][tile W x TP u32]
 static_assert(sizeof(LaneInfo) == kLaneInfoBytes, "LaneInfo size");
@@ -101,11 +107,11 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
   const uint32_t W = 1u << log_w;
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   uint2 *stw = reinterpret_cast<uint2 *>(smem + lane_bytes(log_w));
-  uint32_t *tile = reinterpret_cast<uint32_t *>(smem + lane_bytes(log_w) + (F::ID == 1 ? T * sizeof(uint2) : 0));
+  uint32_t *tile = reinterpret_cast<uint32_t *>(smem + lane_bytes(log_w) + (F::MONTY ? T * sizeof(uint2) : 0));
 
   const uint64_t g0 = tile_idx << log_w;
   for (uint32_t l = threadIdx.x; l < W; l += NT) decode_lane(a, LOG_T, g0 + l, lanes[l]);
-  if constexpr (F::ID == 1) {
+  if constexpr (F::MONTY) {
     const uint2 *gtw = reinterpret_cast<const uint2 *>(a.tw);
     for (uint32_t i = threadIdx.x; i < T - 1; i += NT) stw[i] = gtw[i];
   }
@@ -113,7 +119,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
 
   // ---- load (coalesced), inverse four-step twist on the way in ----
   const uint64_t sps = ((uint64_t)1 << a.lo) * a.src_rs;
-  const bool twist_in = F::ID == 1 && DIR == 1 && a.twist != nullptr;
+  const bool twist_in = F::MONTY && DIR == 1 && a.twist != nullptr;
   for (uint32_t e = threadIdx.x; e < (W << LOG_T); e += NT) {
     uint32_t lane, p;
     if (a.lane_contig) { lane = e & (W - 1); p = e >> log_w; }
@@ -122,8 +128,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
     uint32_t v = 0;
     if (li.valid) {
       v = a.src[li.sbase + p * sps];
-      if constexpr (F::ID == 1 && DIR == 1) {
-        if (twist_in) v = BabyBear::mul_tw(v, a.twist[((uint64_t)p << a.lo) | li.low]);
+      if constexpr (F::MONTY && DIR == 1) {
+        if (twist_in) v = F::mul_tw(v, a.twist[((uint64_t)p << a.lo) | li.low]);
       }
     }
     tile[lane * TP + Tile<LOG_T>::idx(p)] = v;
@@ -131,7 +137,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
   __syncthreads();
 
   TwCtx<F> tw;
-  if constexpr (F::ID == 1) {
+  if constexpr (F::MONTY) {
     tw.stw = stw;
   } else {
     tw.tbl = reinterpret_cast<const uint32_t *>(a.tw);
@@ -144,7 +150,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
 
   // ---- store (coalesced), forward twist / 1/n scale on the way out ----
   const uint64_t dps = ((uint64_t)1 << a.lo) * a.dst_rs;
-  const bool twist_out = F::ID == 1 && DIR == 0 && a.twist != nullptr;
+  const bool twist_out = F::MONTY && DIR == 0 && a.twist != nullptr;
   for (uint32_t e = threadIdx.x; e < (W << LOG_T); e += NT) {
     uint32_t lane, p;
     if (a.lane_contig) { lane = e & (W - 1); p = e >> log_w; }
@@ -152,8 +158,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t tile_idx, 
     const LaneInfo &li = lanes[lane];
     if (!li.valid) continue;
     uint32_t v = tile[lane * TP + Tile<LOG_T>::idx(p)];
-    if constexpr (F::ID == 1 && DIR == 0) {
-      if (twist_out) v = BabyBear::mul_tw(v, a.twist[((uint64_t)p << a.lo) | li.low]);
+    if constexpr (F::MONTY && DIR == 0) {
+      if (twist_out) v = F::mul_tw(v, a.twist[((uint64_t)p << a.lo) | li.low]);
     }
     if (a.has_scale) v = scale_mul<F>(v, a.scale_w, a.scale_q);
     a.dst[li.dbase + p * dps] = v;
@@ -282,13 +288,13 @@ __global__ __launch_bounds__(256) void k_layer(const LayerArgs a) {
   uint32_t x = a.src[c * a.src_cs + r0 * a.src_rs];
   uint32_t y = a.src[c * a.src_cs + r1 * a.src_rs];
   typename F::Tw w;
-  if constexpr (F::ID == 1) {
+  if constexpr (F::MONTY) {
     w = reinterpret_cast<const uint2 *>(a.tw)[(1ull << l) - 1 + (r0 & ((1ull << l) - 1))];
   } else {
     const uint64_t off = (1ull << a.log_n) - (1ull << (a.log_n - l));
     w = reinterpret_cast<const uint32_t *>(a.tw)[off + (r0 >> (l + 1))];
   }
-  constexpr bool GS = (F::ID == 1) == (DIR == 0);
+  constexpr bool GS = F::MONTY == (DIR == 0);
   if constexpr (GS) bfly_gs<F>(x, y, w);
   else bfly_ct<F>(x, y, w);
   if (a.has_scale) {
@@ -418,12 +424,24 @@ hipError_t launch_pass_bb_inv(int log_t, int logr, const PassArgs &a, uint64_t t
   return launch_pass_r<BabyBear, 1>(log_t, logr, a, tiles, s);
 }
 #endif
+#if SFN_IN_PART(6)
+hipError_t launch_pass_kb_fwd(int log_t, int logr, const PassArgs &a, uint64_t tiles, hipStream_t s) {
+  return launch_pass_r<KoalaBear, 0>(log_t, logr, a, tiles, s);
+}
+#endif
+#if SFN_IN_PART(7)
+hipError_t launch_pass_kb_inv(int log_t, int logr, const PassArgs &a, uint64_t tiles, hipStream_t s) {
+  return launch_pass_r<KoalaBear, 1>(log_t, logr, a, tiles, s);
+}
+#endif
 
 #if SFN_IN_PART(4)
 hipError_t launch_pass_m31_fwd(int, int, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_pass_m31_inv(int, int, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_pass_bb_fwd(int, int, const PassArgs &, uint64_t, hipStream_t);
 hipError_t launch_pass_bb_inv(int, int, const PassArgs &, uint64_t, hipStream_t);
+hipError_t launch_pass_kb_fwd(int, int, const PassArgs &, uint64_t, hipStream_t);
+hipError_t launch_pass_kb_inv(int, int, const PassArgs &, uint64_t, hipStream_t);
 
 hipError_t launch_pass(int field, int dir, int log_t, int logr, const PassArgs &a, uint64_t tiles,
                        hipStream_t s) {
@@ -431,6 +449,8 @@ hipError_t launch_pass(int field, int dir, int log_t, int logr, const PassArgs &
   if (tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
   if (field == 0) return dir == 0 ? launch_pass_m31_fwd(log_t, logr, a, tiles, s)
                                   : launch_pass_m31_inv(log_t, logr, a, tiles, s);
+  if (field == 2) return dir == 0 ? launch_pass_kb_fwd(log_t, logr, a, tiles, s)
+                                  : launch_pass_kb_inv(log_t, logr, a, tiles, s);
   return dir == 0 ? launch_pass_bb_fwd(log_t, logr, a, tiles, s) : launch_pass_bb_inv(log_t, logr, a, tiles, s);
 }
 
@@ -470,6 +490,9 @@ hipError_t launch_layer(int field, int dir, const LayerArgs &a, hipStream_t s) {
   if (field == 0) {
     if (dir == 0) hipLaunchKernelGGL((k_layer<M31, 0>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL((k_layer<M31, 1>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
+  } else if (field == 2) {
+    if (dir == 0) hipLaunchKernelGGL((k_layer<KoalaBear, 0>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_layer<KoalaBear, 1>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
   } else {
     if (dir == 0) hipLaunchKernelGGL((k_layer<BabyBear, 0>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL((k_layer<BabyBear, 1>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
@@ -483,6 +506,7 @@ hipError_t launch_pointwise(int field, const uint32_t *a, const uint32_t *b, uin
   uint64_t blocks = (len + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (field == 0) hipLaunchKernelGGL((k_pointwise<M31>), dim3((uint32_t)blocks), dim3(256), 0, s, a, b, out, len);
+  else if (field == 2) hipLaunchKernelGGL((k_pointwise<KoalaBear>), dim3((uint32_t)blocks), dim3(256), 0, s, a, b, out, len);
   else hipLaunchKernelGGL((k_pointwise<BabyBear>), dim3((uint32_t)blocks), dim3(256), 0, s, a, b, out, len);
   return hipGetLastError();
 }
@@ -542,6 +566,8 @@ __global__ __launch_bounds__(NT) void k_multi(const SegDesc *segs, uint32_t nseg
       case 1: multi_dispatch<M31, 1>(log_t, a, local, smem); break;
       case 2: multi_dispatch<BabyBear, 0>(log_t, a, local, smem); break;
       case 3: multi_dispatch<BabyBear, 1>(log_t, a, local, smem); break;
+      case 4: multi_dispatch<KoalaBear, 0>(log_t, a, local, smem); break;
+      case 5: multi_dispatch<KoalaBear, 1>(log_t, a, local, smem); break;
       default: break;
     }
     __syncthreads();  // the next tile reuses the LDS

@@ -70,17 +70,17 @@ struct LayerArgs {
 struct SegDesc {
   PassArgs a;
   uint64_t tile_begin;  // first global tile of this segment (segments sorted)
-  uint32_t kind;        // field * 2 + direction
+  uint32_t kind;        // field * 2 + direction (+ 6 for the tuned kernel when lanes are adjacent)
   uint32_t log_t;       // sub-transform size (register radix min(5, log_t))
 };
 
 constexpr size_t kLaneInfoBytes = 32;  // per-lane decode record kept in LDS
 
-// Dynamic LDS of a pass: [lane records][BabyBear small table][tile, W x (T + T/32)]
+// Dynamic LDS of a pass: [lane records][Montgomery-field small table][tile, W x (T + T/32)]
 inline size_t pass_lds_bytes(int field, int log_t, uint32_t log_w) {
   const size_t T = (size_t)1 << log_t;
   const size_t TP = T + (T >> 5);
-  return (kLaneInfoBytes << log_w) + (field == 1 ? T * 8 : 0) + (TP << log_w) * 4;
+  return (kLaneInfoBytes << log_w) + (field != 0 ? T * 8 : 0) + (TP << log_w) * 4;
 }
 // LDE middle pass: [lane records][inverse + forward small tables][tile]
 inline size_t lde_mid_lds_bytes(int log_t, uint32_t log_w) {

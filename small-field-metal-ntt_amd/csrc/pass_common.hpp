@@ -73,19 +73,19 @@ __device__ __forceinline__ void decode_lane(const PassArgs &a, int log_t, uint64
 }
 
 // ---- twiddle fetch inside a register round ----
-// BabyBear: small position table staged in LDS (layer t at offset 2^t - 1).
+// Montgomery fields: small position table staged in LDS (layer t at offset 2^t - 1).
 // M31: block table in global memory, index (hb << (hi-l-1)) | (p >> (t+1)).
-template <class F>
+template <class F, bool MONTY = F::MONTY>
 struct TwCtx;
 
-template <>
-struct TwCtx<BabyBear> {
+template <class F>
+struct TwCtx<F, true> {
   const uint2 *stw;  // LDS
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const { return stw[(1u << t) - 1 + pos]; }
 };
 
 template <>
-struct TwCtx<M31> {
+struct TwCtx<M31, false> {
   const uint32_t *tbl;  // global, whole transform
   uint32_t n_log, lo, hi;
   uint64_t hb;
@@ -101,24 +101,24 @@ struct TwCtx<M31> {
 template <class F, int DIR, int K, int B0, class TW>
 __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase, const TW &tw) {
   constexpr int R = 1 << K;
-  constexpr bool GS = (F::ID == 1) == (DIR == 0);  // BB fwd and M31 inv are GS
+  constexpr bool GS = F::MONTY == (DIR == 0);  // Montgomery fwd and M31 inv are GS
   if constexpr (DIR == 0) {
 #pragma unroll
     for (int j = K - 1; j >= 0; --j) {
 #pragma unroll
-      for (int m = 0; m < (1 << (F::ID == 1 ? j : (K - 1 - j))); ++m) {
+      for (int m = 0; m < (1 << (F::MONTY ? j : (K - 1 - j))); ++m) {
         typename F::Tw w;
-        if constexpr (F::ID == 1) {
+        if constexpr (F::MONTY) {
           w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
           w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
         }
 #pragma unroll
-        for (int q = 0; q < (1 << (F::ID == 1 ? (K - 1 - j) : j)); ++q) {
+        for (int q = 0; q < (1 << (F::MONTY ? (K - 1 - j) : j)); ++q) {
           // BB: twiddle depends on i mod 2^j -> i = m + q*2^(j+1)
           // M31: twiddle depends on i >> (j+1) -> i = (m << (j+1)) + q
-          const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
-          if constexpr (F::ID == 1 && B0 == 0) {
+          const int i = F::MONTY ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (F::MONTY && B0 == 0) {
             if (m == 0) {  // position 0 of every layer: the twiddle is 1
               bfly_unit<F>(v[i], v[i + (1 << j)]);
               continue;
@@ -133,17 +133,17 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
 #pragma unroll
     for (int j = 0; j < K; ++j) {
 #pragma unroll
-      for (int m = 0; m < (1 << (F::ID == 1 ? j : (K - 1 - j))); ++m) {
+      for (int m = 0; m < (1 << (F::MONTY ? j : (K - 1 - j))); ++m) {
         typename F::Tw w;
-        if constexpr (F::ID == 1) {
+        if constexpr (F::MONTY) {
           w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
           w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
         }
 #pragma unroll
-        for (int q = 0; q < (1 << (F::ID == 1 ? (K - 1 - j) : j)); ++q) {
-          const int i = F::ID == 1 ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
-          if constexpr (F::ID == 1 && B0 == 0) {
+        for (int q = 0; q < (1 << (F::MONTY ? (K - 1 - j) : j)); ++q) {
+          const int i = F::MONTY ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          if constexpr (F::MONTY && B0 == 0) {
             if (m == 0) {
               bfly_unit<F>(v[i], v[i + (1 << j)]);
               continue;

@@ -37,14 +37,14 @@ Context::~Context() {
     b.ptr = nullptr;
     b.bytes = 0;
   };
-  rel(bb_pos[0]);
-  rel(bb_pos[1]);
+  for (auto &f : mf_pos)
+    for (auto &b : f) rel(b);
   rel(bb_small_pair);
-  rel(bb_root);
-  for (auto &kv : bb_twist) rel(kv.second);
+  for (auto &b : mf_root) rel(b);
+  for (auto &kv : mf_twist) rel(kv.second);
   for (auto &kv : bb_lde_mid) rel(kv.second);
   for (auto &kv : m31_tw) rel(kv.second);
-  for (auto &kv : bb_twist_m) rel(kv.second);
+  for (auto &kv : mf_twist_m) rel(kv.second);
   for (auto &kv : bb_lde_fast_tab) {
     rel(kv.second.first);
     rel(kv.second.second);
@@ -124,28 +124,29 @@ hipError_t Context::timing_mark(hipStream_t s, bool before, const std::string &l
 }
 
 // ------------------------------------------------------------------ tables
-sfn_status Context::bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv) {
+sfn_status Context::position_table(int field, uint32_t log_max, const uint2 **fwd, const uint2 **inv) {
   std::lock_guard<std::recursive_mutex> g(mu);
+  const int fi = mf_index(field);
   if (log_max < kMaxLogTile) log_max = kMaxLogTile;
-  if (bb_pos_log < log_max) {
+  if (mf_pos_log[fi] < log_max) {
     for (int d = 0; d < 2; ++d) {
-      auto t = host::bb_position_table(log_max, d == 1);
-      if (bb_pos[d].bytes) table_bytes -= bb_pos[d].bytes;
-      sfn_status s = upload_table(bb_pos[d], t.data(), t.size() * sizeof(host::Pair));
+      auto t = host::position_table(host::monty_field(field), log_max, d == 1);
+      if (mf_pos[fi][d].bytes) table_bytes -= mf_pos[fi][d].bytes;
+      sfn_status s = upload_table(mf_pos[fi][d], t.data(), t.size() * sizeof(host::Pair));
       if (s) return s;
     }
-    bb_pos_log = log_max;
+    mf_pos_log[fi] = log_max;
   }
-  if (fwd) *fwd = (const uint2 *)bb_pos[0].ptr;
-  if (inv) *inv = (const uint2 *)bb_pos[1].ptr;
+  if (fwd) *fwd = (const uint2 *)mf_pos[fi][0].ptr;
+  if (inv) *inv = (const uint2 *)mf_pos[fi][1].ptr;
   return SFN_OK;
 }
 
 sfn_status Context::bb_small_pair_table(const uint2 **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
   if (!bb_small_pair.ptr) {
-    auto f = host::bb_position_table(kMaxLogTile, false);
-    auto i = host::bb_position_table(kMaxLogTile, true);
+    auto f = host::position_table(host::kBabyBear, kMaxLogTile, false);
+    auto i = host::position_table(host::kBabyBear, kMaxLogTile, true);
     std::vector<host::Pair> both(i);
     both.insert(both.end(), f.begin(), f.end());
     sfn_status s = upload_table(bb_small_pair, both.data(), both.size() * sizeof(host::Pair));
@@ -155,28 +156,31 @@ sfn_status Context::bb_small_pair_table(const uint2 **out) {
   return SFN_OK;
 }
 
-sfn_status Context::bb_root_table(const uint2 **out) {
+sfn_status Context::root_table(int field, const uint2 **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
-  if (!bb_root.ptr) {
-    auto t = host::bb_root_table(kMaxLogTile);
-    sfn_status s = upload_table(bb_root, t.data(), t.size() * sizeof(host::Pair));
+  DevBuf &r = mf_root[mf_index(field)];
+  if (!r.ptr) {
+    auto t = host::root_table(host::monty_field(field), kMaxLogTile);
+    sfn_status s = upload_table(r, t.data(), t.size() * sizeof(host::Pair));
     if (s) return s;
   }
-  *out = (const uint2 *)bb_root.ptr;
+  *out = (const uint2 *)r.ptr;
   return SFN_OK;
 }
 
-sfn_status Context::bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, bool low_major, const uint32_t **out) {
+sfn_status Context::twist_m_table(int field, uint32_t lo, uint32_t hi, bool inverse, bool low_major,
+                                  const uint32_t **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(lo, hi, (int)inverse, (int)low_major);
-  auto it = bb_twist_m.find(key);
-  if (it == bb_twist_m.end()) {
-    auto t = host::to_monty_words(host::bb_twist_table(lo, hi, inverse));
+  auto key = std::make_tuple(mf_index(field), lo, hi, (int)inverse, (int)low_major);
+  auto it = mf_twist_m.find(key);
+  if (it == mf_twist_m.end()) {
+    const host::Monty &f = host::monty_field(field);
+    auto t = host::to_monty_words(f, host::twist_table(f, lo, hi, inverse));
     if (low_major) t = host::low_major(t, lo, hi);
     DevBuf b;
     sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
     if (s) return s;
-    it = bb_twist_m.emplace(key, b).first;
+    it = mf_twist_m.emplace(key, b).first;
   }
   *out = (const uint32_t *)it->second.ptr;
   return SFN_OK;
@@ -210,16 +214,16 @@ sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t sh
   return SFN_OK;
 }
 
-sfn_status Context::bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out) {
+sfn_status Context::twist_table(int field, uint32_t lo, uint32_t hi, bool inverse, const uint2 **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(lo, hi, (int)inverse);
-  auto it = bb_twist.find(key);
-  if (it == bb_twist.end()) {
-    auto t = host::bb_twist_table(lo, hi, inverse);
+  auto key = std::make_tuple(mf_index(field), lo, hi, (int)inverse);
+  auto it = mf_twist.find(key);
+  if (it == mf_twist.end()) {
+    auto t = host::twist_table(host::monty_field(field), lo, hi, inverse);
     DevBuf b;
     sfn_status s = upload_table(b, t.data(), t.size() * sizeof(host::Pair));
     if (s) return s;
-    it = bb_twist.emplace(key, b).first;
+    it = mf_twist.emplace(key, b).first;
   }
   *out = (const uint2 *)it->second.ptr;
   return SFN_OK;
@@ -258,10 +262,10 @@ sfn_status Context::m31_table(uint32_t log_n, bool inverse, const uint32_t **out
 // ------------------------------------------------------------------ validation
 // Mirrors the reference's checks: power-of-two sizes via log_n, len == n*batch
 // (implicit in the ABI), log_n <= 30 (src/ntt/metal_r2.rs:78) and BabyBear's
-// two-adicity 27 (src/field/babybear.rs:70).
+// two-adicity 27 (src/field/babybear.rs:70); KoalaBear's two-adicity is 24.
 sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t log_n, uint32_t batch,
                         int layout) {
-  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR)
+  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR && field != SFN_FIELD_KOALABEAR)
     return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
   if (variant < SFN_CT_DIT_R2 || variant > SFN_VARIANT_AUTO)
     return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown variant");
@@ -269,7 +273,7 @@ sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t 
   if (layout != SFN_LAYOUT_SOA && layout != SFN_LAYOUT_ROW_MAJOR)
     return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown layout");
   if (batch == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "batch must be >= 1");
-  const uint32_t max_log = field == SFN_FIELD_M31 ? 30 : 27;
+  const uint32_t max_log = field == SFN_FIELD_M31 ? 30 : host::monty_field(field).max_log;
   if (log_n > max_log) return ctx->fail(SFN_ERR_INVALID_SIZE, "log_n " + std::to_string(log_n) + " exceeds the field limit");
   if (((uint64_t)batch << log_n) > (1ull << 34)) return ctx->fail(SFN_ERR_INVALID_SIZE, "transform too large");
   return SFN_OK;
@@ -422,6 +426,25 @@ std::vector<PassDesc> plan_ntt_passes(int field, int variant, int dir, uint32_t 
   return passes;
 }
 
+// 1/n of the inverse transform: M31 canonical; Montgomery fields as a Shoup
+// pair (w, floor(w * 2^32 / p)) of the standard value, which the kernels'
+// Shoup multiply applies to Montgomery-form data.
+static void scale_factor(int field, uint64_t n, uint32_t *w, uint32_t *q) {
+  if (field == SFN_FIELD_M31) {
+    *w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
+    *q = 0;
+    return;
+  }
+  const uint32_t p = host::monty_field(field).p;
+  const host::Pair pr = host::shoup(host::invmod((uint32_t)(n % p), p), p);
+  *w = pr.w;
+  *q = pr.q;
+}
+
+static const char *field_tag(int field) {
+  return field == SFN_FIELD_M31 ? "m31," : (field == SFN_FIELD_KOALABEAR ? "kb," : "bb,");
+}
+
 sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src, uint32_t *dst,
                           uint64_t src_cs, uint64_t src_rs, uint64_t dst_cs, uint64_t dst_rs, uint32_t batch,
                           uint32_t log_n_real, PassArgs *out) {
@@ -442,14 +465,14 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   a.instances = (uint64_t)batch << (p.log_n - p.log_t);
   a.src_rev_bits = p.src_rev;
   a.dst_rev_bits = p.dst_rev;
-  if (field == SFN_FIELD_BABYBEAR) {
+  if (field != SFN_FIELD_M31) {
     const uint2 *f, *i;
-    sfn_status st = ctx->bb_position(kMaxLogTile, &f, &i);
+    sfn_status st = ctx->position_table(field, kMaxLogTile, &f, &i);
     if (st) return st;
     a.tw = dir == 0 ? f : i;
     if (p.lo > 0) {
       const uint2 *tw;
-      st = ctx->bb_twist_table(p.lo, p.lo + p.log_t, dir == 1, &tw);
+      st = ctx->twist_table(field, p.lo, p.lo + p.log_t, dir == 1, &tw);
       if (st) return st;
       a.twist = tw;
     }
@@ -461,14 +484,7 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   }
   if (p.scale) {
     a.has_scale = 1;
-    const uint64_t n = 1ull << log_n_real;
-    if (field == SFN_FIELD_BABYBEAR) {
-      host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
-      a.scale_w = pr.w;
-      a.scale_q = pr.q;
-    } else {
-      a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
-    }
+    scale_factor(field, 1ull << log_n_real, &a.scale_w, &a.scale_q);
   }
   return SFN_OK;
 }
@@ -490,15 +506,16 @@ static bool fast_offsets_fit(const PassDesc &p, Layout ls, Layout ld) {
 }
 
 // Tuned-kernel fields of a pass's argument block (after make_pass_args):
-// buffer extents, and for BabyBear the root table and the Montgomery twist.
+// buffer extents, and for the Montgomery fields the root table and the
+// Montgomery-word twist.
 static sfn_status fill_fast_args(Context *ctx, int field, int dir, const PassDesc &p, uint64_t src_elems,
                                  uint64_t dst_elems, PassArgs *out) {
   PassArgs &a = *out;
   a.src_elems = src_elems;
   a.dst_elems = dst_elems;
-  if (field != SFN_FIELD_BABYBEAR) return SFN_OK;
+  if (field == SFN_FIELD_M31) return SFN_OK;
   const uint2 *root;
-  sfn_status st = ctx->bb_root_table(&root);
+  sfn_status st = ctx->root_table(field, &root);
   if (st) return st;
   a.tw = root;
   a.twist = nullptr;
@@ -506,7 +523,7 @@ static sfn_status fill_fast_args(Context *ctx, int field, int dir, const PassDes
     const uint32_t *tm;
     // lanes of one workgroup share their low row bits when columns run fastest
     a.tab_low_major = p.order == kColFastest;
-    if ((st = ctx->bb_twist_m_table(p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st;
+    if ((st = ctx->twist_m_table(field, p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st;
     a.twist_m = tm;
   }
   return SFN_OK;
@@ -527,9 +544,9 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     a.layer = p.layer;
     a.batch = batch;
     a.butterflies = (uint64_t)batch << (p.log_n - 1);
-    if (field == SFN_FIELD_BABYBEAR) {
+    if (field != SFN_FIELD_M31) {
       const uint2 *f, *i;
-      sfn_status st = ctx->bb_position(p.log_n, &f, &i);
+      sfn_status st = ctx->position_table(field, p.log_n, &f, &i);
       if (st) return st;
       a.tw = dir == 0 ? f : i;
     } else {
@@ -541,14 +558,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     a.has_scale = 0;
     if (p.scale) {
       a.has_scale = 1;
-      const uint64_t n = 1ull << log_n_real;
-      if (field == SFN_FIELD_BABYBEAR) {
-        host::Pair pr = host::shoup(host::invmod((uint32_t)(n % host::kBBP), host::kBBP), host::kBBP);
-        a.scale_w = pr.w;
-        a.scale_q = pr.q;
-      } else {
-        a.scale_w = host::invmod((uint32_t)(n % host::kM31P), host::kM31P);
-      }
+      scale_factor(field, 1ull << log_n_real, &a.scale_w, &a.scale_q);
     }
     (void)ctx->timing_mark(s, true, "k_layer");
     hipError_t e = launch_layer(field, dir, a, s);
@@ -566,7 +576,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
       fast_offsets_fit(p, ls, ld)) {
     a.xcd_share_log = xcd_share_log(ctx, p);
     if ((st0 = fill_fast_args(ctx, field, dir, p, src_elems, dst_elems, &a))) return st0;
-    (void)ctx->timing_mark(s, true, std::string("k_fast<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
+    (void)ctx->timing_mark(s, true, std::string("k_fast<") + field_tag(field) + (dir ? "inv," : "fwd,") +
                                        std::to_string(p.log_t) + (p.lane_contig ? ",lanes" : ",points") +
                                        ">lo=" + std::to_string(p.lo));
     hipError_t e = launch_fast(field, dir, p.log_t, p.lane_contig != 0, a, p.tiles, s);
@@ -582,7 +592,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     choose_lanes(q, ls.cs == 1 && ls.rs != 1 ? SFN_LAYOUT_ROW_MAJOR : SFN_LAYOUT_SOA, batch, 4);
     return launch_desc(ctx, field, dir, q, src, dst, ls, ld, batch, log_n_real, s);
   }
-  (void)ctx->timing_mark(s, true, std::string("k_pass<") + (field ? "bb," : "m31,") + (dir ? "inv," : "fwd,") +
+  (void)ctx->timing_mark(s, true, std::string("k_pass<") + field_tag(field) + (dir ? "inv," : "fwd,") +
                                      std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
                                      std::to_string(p.lo));
   hipError_t e = launch_pass(field, dir, p.log_t, p.logr, a, p.tiles, s);
@@ -637,7 +647,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   const uint32_t shift_std =
       shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
   timing_begin_sequence(s);
-  Schedule sc = force ? *force : schedule_for(2 /*lde*/, log_n, batch, layout);
+  Schedule sc = force ? *force : schedule_for(SFN_TUNE_LDE, log_n, batch, layout);
   // middle pass covers [lo, log_n); K1 (inverse) and K3 (forward) cover [0, lo),
   // each in one or more passes of at most max_log_t layers
   uint32_t mlt = std::min<uint32_t>(sc.max_log_t, kMaxLogTile);
@@ -730,8 +740,8 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     a.lane_contig = p.lane_contig;
     a.instances = (uint64_t)batch << lo;
     const uint2 *pair, *twi, *twf, *mid;
-    if ((st = bb_twist_table(lo, log_n, true, &twi))) return st;
-    if ((st = bb_twist_table(lo, log_n, false, &twf))) return st;
+    if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, true, &twi))) return st;
+    if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, false, &twf))) return st;
     if ((st = bb_mid_table(log_n, added_bits, shift_std, &mid))) return st;
     a.twist = twi;
     a.twist_fwd = twf;
@@ -748,9 +758,9 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
         fast_offsets_fit(q, mid_ls, mid_ld)) {
       const uint2 *root;
       const uint32_t *tim, *post, *pointf;
-      if ((st = bb_root_table(&root))) return st;
+      if ((st = root_table(SFN_FIELD_BABYBEAR, &root))) return st;
       const bool low_major = q.order == kColFastest;
-      if ((st = bb_twist_m_table(lo, log_n, true, low_major, &tim))) return st;
+      if ((st = twist_m_table(SFN_FIELD_BABYBEAR, lo, log_n, true, low_major, &tim))) return st;
       if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, low_major, &post, &pointf))) return st;
       a.tab_low_major = low_major;
       a.tw = root;
@@ -864,16 +874,23 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
   out.resize(nl);
   for (size_t k = 0; k < nl; ++k) {
     MultiLaunch &L = out[k];
-    // tuned when every pass of this launch fits the tuned kernel
+    // tuned when every pass of this launch fits the tuned kernel and its
+    // Montgomery-field passes share one field (the kernel stages one root table)
     bool tuned = ctx->allow_fast;
+    int mfield = -1;
     std::vector<PassDesc> q(jobs.size());
     for (size_t j = 0; j < jobs.size() && tuned; ++j) {
       if (k >= jp[j].size()) continue;
       const MultiJob &m = jobs[j];
       q[j] = jp[j][k];
+      if (m.field != SFN_FIELD_M31) {
+        if (mfield >= 0 && mfield != m.field) tuned = false;
+        mfield = m.field;
+      }
       const Layout lay{(uint64_t)1 << m.log_n, 1};
       const bool aligned = ((uintptr_t)m.src % 16 == 0) && ((uintptr_t)m.dst % 16 == 0);
-      tuned = choose_lanes_multi(q[j], m.batch) && (q[j].lane_contig || aligned) && fast_offsets_fit(q[j], lay, lay);
+      tuned = tuned && choose_lanes_multi(q[j], m.batch) && (q[j].lane_contig || aligned) &&
+              fast_offsets_fit(q[j], lay, lay);
     }
     L.tuned = tuned;
     size_t tile_bytes = 0;
@@ -893,10 +910,10 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
                                  &d.a)))
           return st;
         if (p.lane_contig) {
-          d.kind += 4;
+          d.kind += 6;
           if (p.log_w < 5) L.xs = std::max<uint32_t>(L.xs, std::min<uint32_t>(3, 5 - p.log_w));
         }
-        if (m.field == SFN_FIELD_BABYBEAR) L.bb_root = d.a.tw;  // the context's root table, same for all
+        if (m.field != SFN_FIELD_M31) L.mf_root = d.a.tw;  // the context's root table of that field
         L.lanes_bytes = std::max<uint32_t>(L.lanes_bytes, (uint32_t)kLaneInfoBytes << p.log_w);
         const size_t T = (size_t)1 << p.log_t;
         tile_bytes = std::max(tile_bytes, ((T + T / 32) << p.log_w) * 4);
@@ -907,7 +924,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
       L.tiles += p.tiles;
       L.segs.push_back(d);
     }
-    if (tuned) L.lds = (L.bb_root ? ((size_t)8 << kMaxLogTile) : 0) + L.lanes_bytes + tile_bytes;
+    if (tuned) L.lds = (L.mf_root ? ((size_t)8 << kMaxLogTile) : 0) + L.lanes_bytes + tile_bytes;
   }
   return SFN_OK;
 }
@@ -927,7 +944,7 @@ uint32_t multi_grid(const Context *ctx, const MultiLaunch &L) {
 
 hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s) {
   if (L.tuned)
-    return launch_multi_fast(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, L.xs, L.bb_root, L.lanes_bytes, s);
+    return launch_multi_fast(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, L.xs, L.mf_root, L.lanes_bytes, s);
   return launch_multi(d_segs, (uint32_t)L.segs.size(), L.tiles, L.lds, grid, s);
 }
 
@@ -1096,7 +1113,8 @@ sfn_status sfn_pointwise_mul(sfn_ctx *ctx, sfn_field field, const uint32_t *a, c
                              size_t len) {
   if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
   Context &c = ctx->impl;
-  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR) return c.fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
+  if (field != SFN_FIELD_M31 && field != SFN_FIELD_BABYBEAR && field != SFN_FIELD_KOALABEAR)
+    return c.fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
   if (len == 0) return SFN_OK;
   if (!a || !b || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
   (void)hipSetDevice(c.device);
@@ -1163,9 +1181,10 @@ sfn_status sfn_twiddles_prepare(sfn_ctx *ctx, sfn_field field, uint32_t log_n) {
   if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
   Context &c = ctx->impl;
   (void)hipSetDevice(c.device);
-  if (field == SFN_FIELD_BABYBEAR) {
-    if (log_n > 27) return c.fail(SFN_ERR_INVALID_SIZE, "log_n exceeds 27");
-    return c.bb_position(log_n, nullptr, nullptr);
+  if (field == SFN_FIELD_BABYBEAR || field == SFN_FIELD_KOALABEAR) {
+    const uint32_t max_log = sfn::host::monty_field(field).max_log;
+    if (log_n > max_log) return c.fail(SFN_ERR_INVALID_SIZE, "log_n exceeds " + std::to_string(max_log));
+    return c.position_table(field, log_n, nullptr, nullptr);
   }
   if (field != SFN_FIELD_M31) return c.fail(SFN_ERR_INVALID_ARGUMENT, "unknown field");
   if (log_n > 30) return c.fail(SFN_ERR_INVALID_SIZE, "log_n exceeds 30");

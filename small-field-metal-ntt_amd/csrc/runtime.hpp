@@ -62,15 +62,18 @@ class Context {
   hipDeviceProp_t props{};
   std::recursive_mutex mu;  // serialises table-cache updates (server threads share it)
 
-  // device-resident tables
-  DevBuf bb_pos[2];     // position tables (fwd, inv), size 2^bb_pos_log pairs
-  uint32_t bb_pos_log = 0;
-  DevBuf bb_small_pair; // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
-  DevBuf bb_root;       // g_{2^11}^e, e < 2^11, Shoup pairs (tuned kernels)
-  std::map<std::tuple<uint32_t, uint32_t, int>, DevBuf> bb_twist;         // (lo, hi, inverse)
+  // device-resident tables; Montgomery-field tables are indexed by mf_index(field)
+  // (0 = BabyBear, 1 = KoalaBear)
+  static int mf_index(int field) { return field == 2 ? 1 : 0; }
+  DevBuf mf_pos[2][2];     // position tables (fwd, inv), size 2^mf_pos_log pairs
+  uint32_t mf_pos_log[2] = {0, 0};
+  DevBuf bb_small_pair;    // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
+  DevBuf mf_root[2];       // g_{2^11}^e, e < 2^11, Shoup pairs (tuned kernels)
+  std::map<std::tuple<int, uint32_t, uint32_t, int>, DevBuf> mf_twist;    // (field, lo, hi, inverse)
   std::map<std::tuple<uint32_t, uint32_t, uint32_t>, DevBuf> bb_lde_mid;  // (L, a, shift_std)
   std::map<std::pair<uint32_t, int>, DevBuf> m31_tw;                      // (L, inverse)
-  std::map<std::tuple<uint32_t, uint32_t, int, int>, DevBuf> bb_twist_m;  // (lo, hi, inverse, low_major), Montgomery words
+  // (field, lo, hi, inverse, low_major), Montgomery words
+  std::map<std::tuple<int, uint32_t, uint32_t, int, int>, DevBuf> mf_twist_m;
   std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, int>, std::pair<DevBuf, DevBuf>>
       bb_lde_fast_tab;  // (L, a, shift, lo, low_major)
   size_t table_bytes = 0;
@@ -106,14 +109,16 @@ class Context {
   sfn_status acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **out);
   void release_lde_scratch(hipStream_t s);
   sfn_status upload_table(DevBuf &b, const void *host, size_t bytes);
-  sfn_status bb_position(uint32_t log_max, const uint2 **fwd, const uint2 **inv);
+  // field: SFN_FIELD_BABYBEAR or SFN_FIELD_KOALABEAR for the Montgomery-field tables
+  sfn_status position_table(int field, uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);
-  sfn_status bb_root_table(const uint2 **out);
+  sfn_status root_table(int field, const uint2 **out);
   // low_major: entry (low << (hi - lo)) | p instead of (p << lo) | low
-  sfn_status bb_twist_m_table(uint32_t lo, uint32_t hi, bool inverse, bool low_major, const uint32_t **out);
+  sfn_status twist_m_table(int field, uint32_t lo, uint32_t hi, bool inverse, bool low_major,
+                           const uint32_t **out);
   sfn_status bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, bool low_major,
                          const uint32_t **post, const uint32_t **pointf);
-  sfn_status bb_twist_table(uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
+  sfn_status twist_table(int field, uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
   sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
   sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);
 
@@ -153,7 +158,7 @@ struct MultiLaunch {
   size_t lds = 0;
   bool tuned = false;  // k_multi_fast (radix-32 tiles) instead of the generic k_multi
   uint32_t xs = 0;     // XCD grouping of tiles (tuned launches)
-  const void *bb_root = nullptr;  // tuned launches with BabyBear segments: the root table
+  const void *mf_root = nullptr;  // tuned launches with Montgomery-field segments: their root table
   uint32_t lanes_bytes = 0;       // tuned launches: lane-record region
 };
 // Pass k of every job goes into launch k.  A launch uses the tuned kernel when

@@ -5,7 +5,7 @@
 //  - max_inflight worker threads each own one HIP stream; a free worker takes
 //    the queue head once its device-memory footprint fits the budget
 //    (admission by HBM footprint), plus queued jobs of the same shape
-//    (continuous batching): M31/BabyBear NTT jobs of one (field, dir, log_n)
+//    (continuous batching): NTT jobs (any field) of one (field, dir, log_n)
 //    run as one multi-job launch per pass, LDE jobs of one shape run back to
 //    back on the worker's stream with a single completion wait;
 //  - host-memory jobs are staged through the worker's device buffers;
@@ -23,6 +23,7 @@
 #include <unordered_map>
 
 #include "runtime.hpp"
+#include "tables.hpp"
 
 using sfn::Context;
 
@@ -283,10 +284,11 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
     in_b = ((size_t)req->batch << req->log_n) * 4;
     out_b = ((size_t)req->batch << (req->log_n + req->added_bits)) * 4;
   } else if (req->kind == SFN_JOB_NTT) {
-    if ((req->field != SFN_FIELD_M31 && req->field != SFN_FIELD_BABYBEAR) ||
+    if ((req->field != SFN_FIELD_M31 && req->field != SFN_FIELD_BABYBEAR && req->field != SFN_FIELD_KOALABEAR) ||
         (req->dir != SFN_FORWARD && req->dir != SFN_INVERSE))
       return SFN_ERR_INVALID_ARGUMENT;
-    if (req->batch == 0 || req->log_n > (req->field == SFN_FIELD_M31 ? 30u : 27u)) return SFN_ERR_INVALID_SIZE;
+    const uint32_t max_log = req->field == SFN_FIELD_M31 ? 30u : sfn::host::monty_field(req->field).max_log;
+    if (req->batch == 0 || req->log_n > max_log) return SFN_ERR_INVALID_SIZE;
     in_b = out_b = ((size_t)req->batch << req->log_n) * 4;
   } else {
     return SFN_ERR_INVALID_ARGUMENT;

@@ -3,41 +3,41 @@
 namespace sfn {
 namespace host {
 
-std::vector<Pair> bb_position_table(uint32_t log_max, bool inverse) {
+std::vector<Pair> position_table(const Monty &f, uint32_t log_max, bool inverse) {
   std::vector<Pair> t(((size_t)1 << log_max) > 0 ? ((size_t)1 << log_max) : 1);
   for (uint32_t s = 0; s < log_max; ++s) {
-    uint32_t g = bb_root(s + 1);
-    if (inverse) g = invmod(g, kBBP);
+    uint32_t g = f.root(s + 1);
+    if (inverse) g = invmod(g, f.p);
     uint32_t w = 1;
     Pair *layer = t.data() + ((size_t)1 << s) - 1;
     for (size_t j = 0; j < ((size_t)1 << s); ++j) {
-      layer[j] = shoup(w, kBBP);
-      w = mulmod(w, g, kBBP);
+      layer[j] = shoup(w, f.p);
+      w = mulmod(w, g, f.p);
     }
   }
   return t;
 }
 
-std::vector<Pair> bb_root_table(uint32_t log) {
+std::vector<Pair> root_table(const Monty &f, uint32_t log) {
   std::vector<Pair> t((size_t)1 << log);
-  const uint32_t g = bb_root(log);
+  const uint32_t g = f.root(log);
   uint32_t w = 1;
   for (size_t e = 0; e < t.size(); ++e) {
-    t[e] = shoup(w, kBBP);
-    w = mulmod(w, g, kBBP);
+    t[e] = shoup(w, f.p);
+    w = mulmod(w, g, f.p);
   }
   return t;
 }
 
-std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse) {
+std::vector<Pair> twist_table(const Monty &f, uint32_t lo, uint32_t hi, bool inverse) {
   const uint32_t lt = hi - lo;
   const uint64_t mask = ((uint64_t)1 << hi) - 1;
   std::vector<uint32_t> pw((size_t)1 << hi);
-  const uint32_t g = bb_root(hi);
+  const uint32_t g = f.root(hi);
   uint32_t w = 1;
   for (size_t e = 0; e < pw.size(); ++e) {
     pw[e] = w;
-    w = mulmod(w, g, kBBP);
+    w = mulmod(w, g, f.p);
   }
   std::vector<Pair> t((size_t)1 << hi);
   for (uint64_t p = 0; p < ((uint64_t)1 << lt); ++p) {
@@ -45,7 +45,7 @@ std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse) {
     for (uint64_t low = 0; low < ((uint64_t)1 << lo); ++low) {
       uint64_t e = (low * br) & mask;
       if (inverse) e = (((uint64_t)1 << hi) - e) & mask;
-      t[(p << lo) | low] = shoup(pw[e], kBBP);
+      t[(p << lo) | low] = shoup(pw[e], f.p);
     }
   }
   return t;
@@ -77,9 +77,9 @@ std::vector<uint32_t> low_major(const std::vector<uint32_t> &t, uint32_t lo, uin
   return r;
 }
 
-std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t) {
+std::vector<uint32_t> to_monty_words(const Monty &f, const std::vector<Pair> &t) {
   std::vector<uint32_t> m(t.size());
-  for (size_t i = 0; i < t.size(); ++i) m[i] = bb_to_monty(t[i].w);
+  for (size_t i = 0; i < t.size(); ++i) m[i] = f.to_monty(t[i].w);
   return m;
 }
 
@@ -91,7 +91,7 @@ void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std,
   const uint32_t blocks = 1u << added_bits;
   const uint32_t inv_n = invmod((uint32_t)(n % kBBP), kBBP);
   const uint32_t gN = bb_root(log_n + added_bits);
-  const std::vector<Pair> tw = bb_twist_table(lo, log_n, false);
+  const std::vector<Pair> tw = twist_table(kBabyBear, lo, log_n, false);
   post.assign(n * blocks, 0);
   pointf.assign(T * blocks, 0);
   for (uint32_t b = 0; b < blocks; ++b) {

@@ -12,6 +12,10 @@ namespace host {
 constexpr uint32_t kM31P = 0x7FFFFFFFu;
 constexpr uint32_t kBBP = 0x78000001u;
 constexpr uint32_t kBBRoot27 = 440564289u;  // primitive 2^27-th root (src/field/babybear.rs:31-33)
+// KoalaBear, p = 2^31 - 2^24 + 1 = 2^24 * 127 + 1 (README.md:20, planned in the
+// reference): 3 generates F_p^*, so 3^127 is a primitive 2^24-th root.
+constexpr uint32_t kKBP = 0x7F000001u;
+constexpr uint32_t kKBRoot24 = 1791270792u;  // 3^127 mod p
 
 inline uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_t)a * b % p); }
 inline uint32_t powmod(uint32_t a, uint64_t e, uint32_t p) {
@@ -35,6 +39,18 @@ inline uint32_t bb_from_monty(uint32_t m) {
 // Primitive 2^k-th root of unity, standard form (two_adic_generator, src/field/babybear.rs:69-76).
 inline uint32_t bb_root(uint32_t k) { return powmod(kBBRoot27, 1ull << (27 - k), kBBP); }
 
+// A Montgomery field (BabyBear or KoalaBear): modulus, primitive 2^max_log-th
+// root (standard form) and two-adicity.  The NTT tables below take one.
+struct Monty {
+  uint32_t p, root_max, max_log;
+  uint32_t root(uint32_t k) const { return powmod(root_max, 1ull << (max_log - k), p); }
+  uint32_t to_monty(uint32_t v) const { return (uint32_t)(((uint64_t)v << 32) % p); }
+};
+constexpr Monty kBabyBear{kBBP, kBBRoot27, 27};
+constexpr Monty kKoalaBear{kKBP, kKBRoot24, 24};
+// field id of the C ABI: 1 = BabyBear, 2 = KoalaBear
+inline const Monty &monty_field(int field) { return field == 2 ? kKoalaBear : kBabyBear; }
+
 // Shoup pair: (w, floor(w * 2^32 / p))
 struct Pair {
   uint32_t w, q;
@@ -50,24 +66,24 @@ inline uint64_t bitrev(uint64_t v, uint32_t bits) {
   return r;
 }
 
-// BabyBear position table (src/field/babybear.rs:83-97; inverse:
+// Position table (BabyBear: src/field/babybear.rs:83-97; inverse:
 // src/ntt/babybear/twiddles.rs:13-19): layer s at offset 2^s - 1 holds
 // g_{s+1}^(+-j), j < 2^s, as Shoup pairs.  Independent of the transform size.
-std::vector<Pair> bb_position_table(uint32_t log_max, bool inverse);
+std::vector<Pair> position_table(const Monty &f, uint32_t log_max, bool inverse);
 
 // Root table g_{2^log}^e, e < 2^log (Shoup pairs): one table for every layer
 // and both directions of a sub-transform of up to 2^log points.
-std::vector<Pair> bb_root_table(uint32_t log);
+std::vector<Pair> root_table(const Monty &f, uint32_t log);
 
-// BabyBear four-step twist of layers [lo, hi): entry (p << lo) | low holds
+// Four-step twist of layers [lo, hi): entry (p << lo) | low holds
 // g_hi^(+-low * bitrev(p, hi - lo)).
-std::vector<Pair> bb_twist_table(uint32_t lo, uint32_t hi, bool inverse);
+std::vector<Pair> twist_table(const Monty &f, uint32_t lo, uint32_t hi, bool inverse);
 
 // Reorders a table of 2^hi entries indexed (p << lo) | low into (low << (hi - lo)) | p.
 std::vector<uint32_t> low_major(const std::vector<uint32_t> &t, uint32_t lo, uint32_t hi);
 
 // Single-word Montgomery form (w * 2^32 mod p) of a table of standard values.
-std::vector<uint32_t> to_monty_words(const std::vector<Pair> &t);
+std::vector<uint32_t> to_monty_words(const Monty &f, const std::vector<Pair> &t);
 
 // Coset-LDE tuned middle pass, sigma_b = shift * g_{L+a}^bitrev(b, a):
 //  post:   block b entry (p << lo) | low = g_L^(low*bitrev(p, L-lo)) * n^-1 * sigma_b^low

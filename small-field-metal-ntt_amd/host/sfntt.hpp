@@ -76,7 +76,7 @@ inline void check(sfn_status s, const sfn_ctx *ctx = nullptr) {
   }
 }
 
-enum class Field : int { M31 = SFN_FIELD_M31, BabyBear = SFN_FIELD_BABYBEAR };
+enum class Field : int { M31 = SFN_FIELD_M31, BabyBear = SFN_FIELD_BABYBEAR, KoalaBear = SFN_FIELD_KOALABEAR };
 enum class Variant : int {
   CtDitR2 = SFN_CT_DIT_R2,
   CtGsR2 = SFN_CT_GS_R2,
@@ -174,7 +174,7 @@ class Backend {
  protected:
   GpuResult gpu(sfn_direction dir, const std::vector<uint32_t> &input, uint32_t log_n, uint32_t batch,
                 Layout layout) const {
-    const uint32_t max_log = field_ == Field::M31 ? 30u : 27u;
+    const uint32_t max_log = field_ == Field::M31 ? 30u : (field_ == Field::KoalaBear ? 24u : 27u);
     if (batch == 0 || log_n > max_log || input.size() != (size_t(batch) << log_n))
       throw InvalidSize(SFN_ERR_INVALID_SIZE, std::to_string(input.size()));
     if (log_n == 0) return {input, 0};
@@ -194,6 +194,8 @@ class BbBatchNtt : public Backend {
  public:
   explicit BbBatchNtt(std::shared_ptr<Context> ctx)
       : Backend(std::move(ctx), Field::BabyBear, Variant::Auto, "bb-hip-batch") {}
+  BbBatchNtt(std::shared_ptr<Context> ctx, Field f, std::string name)
+      : Backend(std::move(ctx), f, Variant::Auto, std::move(name)) {}
   GpuResult forward_ntt_batch_gpu(const std::vector<uint32_t> &data, uint32_t log_n, uint32_t batch,
                                   Layout layout = Layout::SoA) const {
     return gpu(SFN_FORWARD, data, log_n, batch, layout);
@@ -214,6 +216,17 @@ inline Backend BbMetalCtDitR2(std::shared_ptr<Context> c) { return Backend(std::
 inline Backend BbMetalR2(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::BabyBear, Variant::CtGsR2, "bb-hip-r2"); }
 inline Backend BbMetalStockhamR2(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::BabyBear, Variant::StockhamR2, "bb-hip-stockham-r2"); }
 inline Backend BbMetalCtGsR4(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::BabyBear, Variant::CtGsR4, "bb-hip-ct-gs-r4"); }
+
+// KoalaBear (README.md:20, planned in the reference): the BabyBear variants'
+// conventions with p = 2^31 - 2^24 + 1 and two-adicity 24.
+inline Backend KbMetalCtDitR2(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::KoalaBear, Variant::CtDitR2, "kb-hip-ct-dit-r2"); }
+inline Backend KbMetalR2(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::KoalaBear, Variant::CtGsR2, "kb-hip-r2"); }
+inline Backend KbMetalStockhamR2(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::KoalaBear, Variant::StockhamR2, "kb-hip-stockham-r2"); }
+inline Backend KbMetalCtGsR4(std::shared_ptr<Context> c) { return Backend(std::move(c), Field::KoalaBear, Variant::CtGsR4, "kb-hip-ct-gs-r4"); }
+class KbBatchNtt : public BbBatchNtt {
+ public:
+  explicit KbBatchNtt(std::shared_ptr<Context> ctx) : BbBatchNtt(std::move(ctx), Field::KoalaBear, "kb-hip-batch") {}
+};
 
 // Device-resident twiddle tables keyed by (field, log_n); the reference's
 // TwiddleCache keeps only the last log_n on the host.

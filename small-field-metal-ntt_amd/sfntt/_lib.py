@@ -16,7 +16,8 @@ OK = 0
 ERR_INVALID_SIZE, ERR_DEVICE_NOT_FOUND, ERR_BUFFER_ALLOC_FAILED = 1, 2, 3
 ERR_KERNEL_LOAD, ERR_GPU_EXECUTION, ERR_CORRECTNESS, ERR_UNSUPPORTED = 4, 5, 6, 7
 ERR_INVALID_ARGUMENT, ERR_BUSY, ERR_NOT_READY = 8, 9, 10
-FIELD_M31, FIELD_BABYBEAR = 0, 1
+FIELD_M31, FIELD_BABYBEAR, FIELD_KOALABEAR = 0, 1, 2
+TUNE_LDE = 16  # sfn_autotune's field value for the coset LDE
 CT_DIT_R2, CT_GS_R2, STOCKHAM_R2, CT_GS_R4, VARIANT_AUTO = 0, 1, 2, 3, 4
 FORWARD, INVERSE = 0, 1
 LAYOUT_SOA, LAYOUT_ROW_MAJOR = 0, 1

@@ -288,10 +288,11 @@ class TunedM31(_GpuBackend):
 class _Bb(_GpuBackend):
     FIELD = L.FIELD_BABYBEAR
     MODULUS = 0x78000001
+    MAX_LOG = 27  # two-adicity (src/field/babybear.rs:70)
 
     def _gpu(self, direction, data, log_n):
         a = _u32(data)
-        if a.size != (1 << log_n) or log_n > 27:
+        if a.size != (1 << log_n) or log_n > self.MAX_LOG:
             raise InvalidSize(a.size)
         if log_n == 0:
             return a.copy(), 0
@@ -320,7 +321,7 @@ class BbBatchNtt(_Bb):
 
     def _batch(self, direction, data, log_n, batch_size, layout):
         a = _u32(data)
-        if a.size != (batch_size << log_n) or log_n > 27 or batch_size == 0:
+        if a.size != (batch_size << log_n) or log_n > self.MAX_LOG or batch_size == 0:
             raise InvalidSize(a.size)
         if log_n == 0:
             return a.copy(), 0
@@ -331,6 +332,37 @@ class BbBatchNtt(_Bb):
 
     def inverse_ntt_batch_gpu(self, data, log_n: int, batch_size: int, layout=L.LAYOUT_SOA):
         return self._batch(L.INVERSE, data, log_n, batch_size, layout)
+
+
+# ---- KoalaBear variants (README.md:20 lists the field as planned) ----
+# Same conventions as BabyBear: Montgomery raw values (R = 2^32), forward from
+# natural order to bit-reversed evaluations, inverse scaled by 1/n; p = 2^31 -
+# 2^24 + 1, two-adicity 24.
+class _Kb(_Bb):
+    FIELD = L.FIELD_KOALABEAR
+    MODULUS = 0x7F000001
+    MAX_LOG = 24
+
+
+class KbCtDitR2(_Kb):
+    NAME, VARIANT = "kb-hip-ct-dit-r2", L.CT_DIT_R2
+
+
+class KbR2(_Kb):
+    NAME, VARIANT = "kb-hip-r2", L.CT_GS_R2
+
+
+class KbStockhamR2(_Kb):
+    NAME, VARIANT = "kb-hip-stockham-r2", L.STOCKHAM_R2
+
+
+class KbCtGsR4(_Kb):
+    NAME, VARIANT = "kb-hip-ct-gs-r4", L.CT_GS_R4
+
+
+class KbBatchNtt(_Kb, BbBatchNtt):
+    """Batched KoalaBear NTT, SoA layout data[col * n + row] (BbBatchNtt's API)."""
+    NAME, VARIANT = "kb-hip-batch", L.VARIANT_AUTO
 
 
 class LdeResult:
@@ -372,6 +404,8 @@ class CosetLdeBatch:
 # reference type names
 MetalCtDitR2, MetalCtGsR2, MetalStockhamR2, MetalCtGsR4 = CtDitR2, CtGsR2, StockhamR2, CtGsR4
 BbMetalCtDitR2, BbMetalR2, BbMetalStockhamR2, BbMetalCtGsR4 = BbCtDitR2, BbR2, BbStockhamR2, BbCtGsR4
+KbMetalCtDitR2, KbMetalR2, KbMetalStockhamR2, KbMetalCtGsR4 = KbCtDitR2, KbR2, KbStockhamR2, KbCtGsR4
 
 M31_VARIANTS = [CtDitR2, CtGsR2, StockhamR2, CtGsR4, TunedM31]
 BB_VARIANTS = [BbCtDitR2, BbR2, BbStockhamR2, BbCtGsR4, BbBatchNtt]
+KB_VARIANTS = [KbCtDitR2, KbR2, KbStockhamR2, KbCtGsR4, KbBatchNtt]

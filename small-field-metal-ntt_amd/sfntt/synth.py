@@ -13,6 +13,7 @@ import numpy as np
 A = np.uint64(6364136223846793005)
 M31_P = 0x7FFFFFFF
 BB_P = 0x78000001
+KB_P = 0x7F000001
 _CHUNK = 1 << 20
 _tables = None
 
@@ -49,8 +50,8 @@ def lcg_u32(seed: int, n: int, modulus: int, monty: bool, out: np.ndarray | None
 
 
 def field_data(field: int, seed: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
-    """field 0 = M31 (canonical), 1 = BabyBear (Montgomery raw)."""
-    return lcg_u32(seed, n, M31_P if field == 0 else BB_P, field == 1, out)
+    """field 0 = M31 (canonical), 1 = BabyBear, 2 = KoalaBear (Montgomery raw)."""
+    return lcg_u32(seed, n, (M31_P, BB_P, KB_P)[field], field != 0, out)
 
 
 def splitmix64(state: int):

@@ -47,6 +47,50 @@ def bb_forward_closed(raw: list[int]) -> list[int]:
     return [to_monty(X[bitrev(i, L)]) for i in range(n)]
 
 
+# ------------------------------------------------------------ KoalaBear
+# p = 2^31 - 2^24 + 1, p - 1 = 2^24 * 127; 3 generates F_p^*, so 3^127 is a
+# primitive 2^24-th root of unity.
+KB_P = (1 << 31) - (1 << 24) + 1
+KB_GEN = 3
+
+
+def kb_root(log_n: int) -> int:
+    """Primitive 2^log_n-th root of unity (standard form), derived from the
+    multiplicative generator rather than from any table."""
+    return pow(KB_GEN, (KB_P - 1) >> log_n, KB_P)
+
+
+def kb_to_monty(v: int) -> int:
+    return (v * R) % KB_P
+
+
+def kb_from_monty(m: int) -> int:
+    return (m * pow(R, -1, KB_P)) % KB_P
+
+
+def kb_forward_closed(raw: list[int]) -> list[int]:
+    """KoalaBear forward NTT in the BabyBear convention: out[i] = X[bitrev(i)],
+    X_k = sum_j x_j w^(jk), Montgomery raw in and out."""
+    n = len(raw)
+    L = n.bit_length() - 1
+    x = [kb_from_monty(v) for v in raw]
+    w = kb_root(L)
+    X = [sum(x[j] * pow(w, j * k, KB_P) for j in range(n)) % KB_P for k in range(n)]
+    return [kb_to_monty(X[bitrev(i, L)]) for i in range(n)]
+
+
+def kb_inverse_closed(raw: list[int]) -> list[int]:
+    """Inverse of kb_forward_closed: bit-reversed evaluations in, coefficients
+    out, x_j = n^-1 sum_k y[bitrev(k)] w^(-jk)."""
+    n = len(raw)
+    L = n.bit_length() - 1
+    y = [kb_from_monty(v) for v in raw]
+    winv = pow(kb_root(L), -1, KB_P)
+    ninv = pow(n, -1, KB_P)
+    x = [ninv * sum(y[bitrev(k, L)] * pow(winv, j * k, KB_P) for k in range(n)) % KB_P for j in range(n)]
+    return [kb_to_monty(v) for v in x]
+
+
 def coset_lde_closed(col_raw: list[int], added_bits: int, shift_std: int | None) -> list[int]:
     """Reference LDE semantics for one column: the input holds evaluations of a
     degree < n polynomial P over H_n in bit-reversed order; the output holds P

@@ -43,6 +43,12 @@ std::vector<uint32_t> lcg(uint64_t seed, size_t n, bool babybear) {
   return v;
 }
 
+std::vector<uint32_t> lcg_kb(uint64_t seed, size_t n) {
+  std::vector<uint32_t> v(n);
+  orc_lcg_fill(seed, ORC_KB_P, 2, v.data(), n);
+  return v;
+}
+
 int no_device() {
   using namespace sfntt;
   expect(throws<InvalidSize>([] { check(SFN_ERR_INVALID_SIZE); }), "check(INVALID_SIZE) throws InvalidSize");
@@ -104,6 +110,30 @@ int parity() {
       GpuResult r = b.forward_ntt_gpu(x, log_n);
       expect(r.first == want, b.name() + " forward_ntt_gpu 2^" + std::to_string(log_n));
     }
+  }
+  // KoalaBear variants and batch
+  std::vector<Backend> kb = {KbMetalCtDitR2(ctx), KbMetalR2(ctx), KbMetalStockhamR2(ctx), KbMetalCtGsR4(ctx)};
+  for (uint32_t log_n : {5u, 13u, 17u}) {
+    const size_t n = size_t(1) << log_n;
+    const std::vector<uint32_t> x = lcg_kb(777 + n, n);
+    std::vector<uint32_t> want = x;
+    orc_kb_forward(want.data(), n);
+    for (const Backend &b : kb) {
+      std::vector<uint32_t> y = x;
+      b.forward_ntt(y);
+      expect(y == want, b.name() + " forward 2^" + std::to_string(log_n));
+      b.inverse_ntt(y);
+      expect(y == x, b.name() + " inverse(forward) 2^" + std::to_string(log_n));
+    }
+  }
+  {
+    KbBatchNtt batch(ctx);
+    const std::vector<uint32_t> x = lcg_kb(31, size_t(5) << 11);
+    std::vector<uint32_t> want = x;
+    orc_ntt_batch(2, 1, 11, 5, 0, want.data(), 1);
+    expect(batch.inverse_ntt_batch_gpu(x, 11, 5).first == want, "KbBatchNtt inverse SoA");
+    expect(throws<InvalidSize>([&] { batch.forward_ntt_batch_gpu(std::vector<uint32_t>(size_t(1) << 25), 25, 1); }),
+           "KoalaBear beyond two-adicity 24 throws InvalidSize");
   }
   // batched BabyBear, SoA and row-major
   {

@@ -102,6 +102,23 @@ def test_plan_of_contiguous_small_columns(ctx, in_place):
         assert np.array_equal(y[o:o + (1 << ln)], exp), f"column at {o} field={f} log_n={ln} dir={d}"
 
 
+@pytest.mark.parametrize("mix", ["koalabear_only", "all_fields"])
+def test_plan_with_koalabear_jobs(ctx, mix):
+    # KoalaBear jobs alone run on the tuned many-job kernel with KoalaBear's
+    # root table; mixed with BabyBear (two root tables) the plan takes the
+    # generic many-job kernel.  Every column equals its single-column transform.
+    st = orc.splitmix64_stream(0x6B0A1A)
+    specs, dirs = [], []
+    for k in range(40):
+        f = 2 if mix == "koalabear_only" else int(next(st) % 3)
+        specs.append((f, int(8 + next(st) % 13), int(next(st))))
+        dirs.append(k % 2)
+    inputs, outs, launches = run_plan(ctx, specs, 1, dirs)
+    assert launches <= 3
+    for (f, log_n, s), d, x, y in zip(specs, dirs, inputs, outs):
+        assert np.array_equal(y, orc.ntt_batch(f, d == 1, log_n, 1, x)), (f, log_n, d)
+
+
 def test_graph_replays_each_match(ctx):
     # replay several times with host work between replays; every replay is a
     # fresh forward of the previous output, compared with the oracle each time
@@ -218,6 +235,32 @@ def test_serving_results_same_alone_and_in_mix(ctx, device_ptrs):
     for kind, log_n, batch, a, x, out, din in keep:
         got = out.download() if device_ptrs else out
         assert np.array_equal(got, expected(kind, log_n, batch, a, x)), (kind, log_n, batch, a)
+
+
+def test_serving_koalabear_ntt_jobs(ctx):
+    # KoalaBear NTT jobs of both directions through the serving loop (same-shape
+    # jobs are grouped into one many-job launch per pass); a KoalaBear LDE is
+    # refused (the coset LDE is BabyBear's) and 2^25 exceeds the two-adicity.
+    srv = make_server(ctx, inflight=2)
+    keep = []
+    for i in range(24):
+        log_n, d = 10 + i % 5, i % 2
+        x = synth.field_data(2, 300 + i, 1 << log_n)
+        out = np.zeros_like(x)
+        r = L.Request(L.JOB_NTT, L.FIELD_KOALABEAR, d, log_n, 1, 0, 0, 0, x.ctypes.data, out.ctypes.data)
+        t = ctypes.c_uint64()
+        ctx.check(ctx._lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+        keep.append((t.value, log_n, d, x, out))
+    for t, log_n, d, x, out in keep:
+        ctx.check(ctx._lib.sfn_server_wait(srv, t, None))
+        assert np.array_equal(out, orc.ntt_batch(2, d == 1, log_n, 1, x)), (log_n, d)
+    tb = ctypes.c_uint64()
+    x = np.zeros(1 << 10, np.uint32)
+    lde = L.Request(L.JOB_LDE, L.FIELD_KOALABEAR, 0, 10, 1, 1, 0, 0, x.ctypes.data, x.ctypes.data)
+    assert ctx._lib.sfn_server_submit(srv, ctypes.byref(lde), ctypes.byref(tb)) == L.ERR_UNSUPPORTED
+    big = L.Request(L.JOB_NTT, L.FIELD_KOALABEAR, 0, 25, 1, 0, 0, 0, x.ctypes.data, x.ctypes.data)
+    assert ctx._lib.sfn_server_submit(srv, ctypes.byref(big), ctypes.byref(tb)) == L.ERR_INVALID_SIZE
+    ctx._lib.sfn_server_destroy(srv)
 
 
 def test_serving_admission_and_errors(ctx):

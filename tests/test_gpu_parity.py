@@ -16,7 +16,7 @@ from sfntt import api
 
 pytestmark = pytest.mark.gpu
 
-M31, BB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR
+M31, BB, KB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR, sfntt.FIELD_KOALABEAR
 VARIANTS = [sfntt.CT_DIT_R2, sfntt.CT_GS_R2, sfntt.STOCKHAM_R2, sfntt.CT_GS_R4, sfntt.VARIANT_AUTO]
 VNAMES = ["v1_ct_dit_r2", "v2_ct_gs_r2", "v3_stockham_r2", "v4_ct_gs_r4", "auto"]
 
@@ -28,11 +28,11 @@ def ctx():
 
 
 def ref_fwd(field, x):
-    return orc.m31_forward(x) if field == M31 else orc.bb_forward(x)
+    return [orc.m31_forward, orc.bb_forward, orc.kb_forward][field](x)
 
 
 def ref_inv(field, x):
-    return orc.m31_inverse(x) if field == M31 else orc.bb_inverse(x)
+    return [orc.m31_inverse, orc.bb_inverse, orc.kb_inverse][field](x)
 
 
 def data(field, n, seed):
@@ -48,7 +48,7 @@ def assert_same(got, exp, what):
         raise AssertionError(f"{what}: {bad.size} mismatches, first at {i}: got {got[i]} expected {exp[i]}")
 
 
-@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+@pytest.mark.parametrize("field", [M31, BB, KB], ids=["m31", "bb", "kb"])
 @pytest.mark.parametrize("variant", VARIANTS, ids=VNAMES)
 def test_single_column_sizes(ctx, field, variant):
     for log_n in list(range(1, 15)) + [16]:
@@ -86,7 +86,26 @@ def test_config2_m31_2e20_all_variants(ctx):
         assert_same(back, x, b.name() + " roundtrip")
 
 
-@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+def test_koalabear_2e16_all_variants_and_delta(ctx):
+    # KoalaBear has no reference code (README.md:20 lists it as planned): the
+    # oracle is pinned by closed forms; here every variant matches it at 2^16,
+    # and a delta at row 0 transforms to all ones (X_k = 1 for every k).
+    n = 1 << 16
+    x = orc.lcg_fill(12345 + n, n, KB)
+    exp = orc.kb_forward(x)
+    for cls in api.KB_VARIANTS:
+        b = cls(ctx=ctx)
+        got, _ = b.forward_ntt_gpu(x, 16)
+        assert_same(got, exp, b.name())
+        back, _ = b.inverse_ntt_gpu(got, 16)
+        assert_same(back, x, b.name() + " roundtrip")
+    d = np.zeros(1 << 12, np.uint32)
+    d[0] = orc.kb_to_monty(1)
+    y, _ = ctx.ntt(KB, sfntt.VARIANT_AUTO, sfntt.FORWARD, 12, 1, d)
+    assert (y == orc.kb_to_monty(1)).all()
+
+
+@pytest.mark.parametrize("field", [M31, BB, KB], ids=["m31", "bb", "kb"])
 @pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
 def test_batches_match_single_columns(ctx, field, layout):
     for log_n, batch in [(4, 3), (10, 5), (12, 16), (17, 4), (20, 2)]:
@@ -126,7 +145,7 @@ def test_coset_lde_small(ctx, log_n, batch, added_bits, shift, layout):
 
 def test_edge_cases(ctx):
     # src/ntt/test_utils.rs:110-168
-    for cls in api.M31_VARIANTS + api.BB_VARIANTS:
+    for cls in api.M31_VARIANTS + api.BB_VARIANTS + api.KB_VARIANTS:
         b = cls(ctx=ctx)
         z = np.zeros(64, np.uint32)
         b.forward_ntt(z)
@@ -145,6 +164,10 @@ def test_edge_cases(ctx):
             b.forward_ntt_gpu(np.ones(8, np.uint32), 4)
     with pytest.raises(api.InvalidSize):
         ctx.ntt(BB, sfntt.VARIANT_AUTO, sfntt.FORWARD, 28, 1, np.zeros(1, np.uint32))
+    with pytest.raises(api.InvalidSize):  # KoalaBear two-adicity 24
+        ctx.ntt(KB, sfntt.VARIANT_AUTO, sfntt.FORWARD, 25, 1, np.zeros(1 << 25, np.uint32))
+    with pytest.raises(api.InvalidSize):
+        api.KbR2(ctx=ctx).forward_ntt_gpu(np.zeros(1 << 25, np.uint32), 25)
     with pytest.raises(api.InvalidSize):
         sfntt.CosetLdeBatch(ctx=ctx).execute(np.zeros(10, np.uint32), 4, 1, 1)
     # M31 delta -> all ones (tests/ntt_cross_validation.rs:355-365)
@@ -173,11 +196,11 @@ def test_pointwise_mul_and_poly_product(ctx):
     bb.inverse_ntt(C)
     assert [orc.bb_from_monty(int(v)) for v in C] == [3, 10, 8, 0]
     rng = np.random.default_rng(5)
-    for field, p in ((M31, orc.M31_P), (BB, orc.BB_P)):
+    for field, p in ((M31, orc.M31_P), (BB, orc.BB_P), (KB, orc.KB_P)):
         a = rng.integers(0, p, 100_003, dtype=np.uint64).astype(np.uint32)
         c = rng.integers(0, p, 100_003, dtype=np.uint64).astype(np.uint32)
         got = ctx.pointwise_mul(field, a, c)
-        mul = orc.m31_mul if field == M31 else orc.bb_mul
+        mul = [orc.m31_mul, orc.bb_mul, orc.kb_mul][field]
         idx = rng.integers(0, a.size, 500)
         assert all(int(got[i]) == mul(int(a[i]), int(c[i])) for i in idx)
 
@@ -203,25 +226,27 @@ def test_determinism_and_device_tier(ctx):
     assert ctx.twiddles_bytes() > 0
 
 
-@pytest.mark.parametrize("field", [M31, BB], ids=["m31", "bb"])
+@pytest.mark.parametrize("field", [M31, BB, KB], ids=["m31", "bb", "kb"])
 def test_large_single_transforms(ctx, field):
     # Single columns beyond one pass pair (2^23..2^26): the four-step split in
-    # three passes.  2^23 against the oracle; 2^24..2^26 by the inverse round
-    # trip and, for BabyBear, out[0] = X_0 = sum of the inputs (mod p).
+    # three passes.  2^23 against the oracle; 2^24..2^26 (KoalaBear: 2^24, its
+    # largest) by the inverse round trip and, for the Montgomery fields,
+    # out[0] = X_0 = sum of the inputs (mod p).
     n = 1 << 23
     x = data(field, n, 2323)
     print(f"[parity] large field={field} log_n=23", flush=True)
     y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.FORWARD, 23, 1, x)
     assert_same(y, ref_fwd(field, x), f"forward 2^23 field={field}")
-    for log_n in (24, 26):
+    for log_n in ((24,) if field == KB else (24, 26)):
         n = 1 << log_n
         x = data(field, n, 1000 + log_n)
         print(f"[parity] large field={field} log_n={log_n}", flush=True)
         y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, 1, x)
         z, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.INVERSE, log_n, 1, y)
         assert_same(z, x, f"round trip 2^{log_n} field={field}")
-        if field == BB:
-            assert int(y[0]) == int(x.astype(np.uint64).sum() % orc.BB_P), f"X_0 at 2^{log_n}"
+        if field != M31:
+            p = orc.BB_P if field == BB else orc.KB_P
+            assert int(y[0]) == int(x.astype(np.uint64).sum() % p), f"X_0 at 2^{log_n}"
 
 
 @pytest.mark.parametrize("log_n,batch,added_bits", [(1, 3, 1), (6, 5, 2), (12, 8, 1), (15, 24, 1), (20, 2, 1),

@@ -18,7 +18,7 @@ from sfntt import _lib as L
 
 pytestmark = pytest.mark.gpu
 
-M31, BB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR
+M31, BB, KB = sfntt.FIELD_M31, sfntt.FIELD_BABYBEAR, sfntt.FIELD_KOALABEAR
 
 
 def context_with(env: dict):
@@ -40,7 +40,8 @@ def ctx():
     return sfntt.HipContext(0)
 
 
-CASES_NTT = [(M31, 12, 1), (M31, 17, 3), (M31, 20, 1), (BB, 11, 5), (BB, 16, 4), (BB, 20, 2)]
+CASES_NTT = [(M31, 12, 1), (M31, 17, 3), (M31, 20, 1), (BB, 11, 5), (BB, 16, 4), (BB, 20, 2), (KB, 13, 3),
+             (KB, 20, 2)]
 CASES_LDE = [(10, 24, 1), (14, 16, 2), (16, 8, 1)]
 
 
@@ -69,6 +70,11 @@ def reference_outputs(ctx):
     x = orc.lcg_fill(1000 + 16, 4 << 16, BB)
     exp = orc.ntt_batch(BB, False, 16, 4, x)
     assert np.array_equal(outs[4 * 4], exp), "default path differs from the oracle"
+    for i, (field, log_n, batch) in enumerate(CASES_NTT):
+        if field == KB:  # KoalaBear: forward and inverse, SoA
+            x = orc.lcg_fill(1000 + log_n, batch << log_n, KB)
+            assert np.array_equal(outs[4 * i], orc.ntt_batch(KB, False, log_n, batch, x)), "KoalaBear forward"
+            assert np.array_equal(outs[4 * i + 1], orc.ntt_batch(KB, True, log_n, batch, x)), "KoalaBear inverse"
     x = orc.lcg_fill(42 + 14, 16 << 14, BB)
     assert np.array_equal(outs[4 * len(CASES_NTT) + 2], orc.coset_lde_batch(x, 14, 16, 2)), "default LDE differs"
     return outs
@@ -91,9 +97,9 @@ def test_autotuned_schedules_give_same_bits(ctx, reference_outputs):
     for field, log_n, batch in CASES_NTT:
         for layout in (sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR):
             c.check(c._lib.sfn_autotune(c.ptr, field, log_n, batch, layout, ctypes.byref(best)))
-    for log_n, batch, _ in CASES_LDE:  # field 2 = the LDE (2x)
+    for log_n, batch, _ in CASES_LDE:  # the LDE (2x)
         for layout in (sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR):
-            c.check(c._lib.sfn_autotune(c.ptr, 2, log_n, batch, layout, ctypes.byref(best)))
+            c.check(c._lib.sfn_autotune(c.ptr, sfntt._lib.TUNE_LDE, log_n, batch, layout, ctypes.byref(best)))
     got = _run_all(c)
     for i, (g, r) in enumerate(zip(got, reference_outputs)):
         assert np.array_equal(g, r), f"case {i} differs under autotuned schedules"

@@ -93,6 +93,67 @@ def test_bb_twiddle_tables(orc):
                 assert orc.bb_from_monty(int(iv[(1 << s) - 1 + j])) == pow(g, -j, BB_P)
 
 
+# -------------------------------------------------------------- KoalaBear
+# The reference has no KoalaBear code (README.md:20 lists it as planned), so
+# the oracle's KoalaBear functions are pinned by closed forms alone:
+# big-integer field arithmetic, root orders, Plonky3's published 2^24-th root,
+# and the brute-force DFT.
+KB_P = cf.KB_P
+
+
+def test_kb_field_closed_forms(orc):
+    assert KB_P == 0x7F000001 and (KB_P - 1) == (1 << 24) * 127
+    assert (KB_P * 0x7EFFFFFF + 1) % (1 << 32) == 0  # -p^-1 mod 2^32
+    rng = random.Random(3)
+    rinv = pow(1 << 32, -1, KB_P)
+    vals = [0, 1, 2, KB_P - 1, KB_P - 2, (1 << 31) - 1 - (1 << 24)] + [rng.randrange(KB_P) for _ in range(2000)]
+    for a, b in zip(vals, reversed(vals)):
+        assert orc.kb_mul(a, b) == a * b * rinv % KB_P
+        assert orc.kb_add(a, b) == (a + b) % KB_P
+        assert orc.kb_sub(a, b) == (a - b) % KB_P
+        assert orc.kb_to_monty(a) == a * (1 << 32) % KB_P
+        assert orc.kb_from_monty(orc.kb_to_monty(a)) == a
+        if a:
+            assert orc.kb_mul(orc.kb_inv(a), a) == orc.kb_to_monty(1)
+    assert orc.kb_monty_reduce((KB_P - 1) * (KB_P - 1)) == (KB_P - 1) ** 2 * rinv % KB_P
+
+
+def test_kb_two_adic_generators(orc):
+    for k in range(1, 25):
+        g = orc.kb_from_monty(orc.kb_two_adic_generator(k))
+        assert g == cf.kb_root(k)
+        assert pow(g, 1 << k, KB_P) == 1 and pow(g, 1 << (k - 1), KB_P) != 1
+    # Plonky3's KoalaBear TWO_ADIC_GENERATOR (koala-bear/src/koala_bear.rs) is 0x6ac49f88
+    assert orc.kb_from_monty(orc.kb_two_adic_generator(24)) == 0x6AC49F88
+    assert orc.kb_two_adic_generator(25) == 0
+
+
+def test_kb_twiddle_tables(orc):
+    for log_n in (1, 4, 9):
+        fw, iv = orc.kb_twiddles(log_n), orc.kb_twiddles(log_n, inverse=True)
+        for s in range(log_n):
+            g = cf.kb_root(s + 1)
+            for j in range(1 << s):
+                assert orc.kb_from_monty(int(fw[(1 << s) - 1 + j])) == pow(g, j, KB_P)
+                assert orc.kb_from_monty(int(iv[(1 << s) - 1 + j])) == pow(g, -j, KB_P)
+
+
+@pytest.mark.parametrize("log_n", range(0, 8))
+def test_kb_forward_and_inverse_closed_forms(orc, log_n):
+    n = 1 << log_n
+    x = orc.lcg_fill(2000 + log_n, n, orc.FIELD_KB)
+    assert int(x.max(initial=0)) < KB_P
+    assert orc.kb_forward(x).tolist() == cf.kb_forward_closed(x.tolist())
+    assert orc.kb_inverse(x).tolist() == cf.kb_inverse_closed(x.tolist())
+
+
+def test_kb_size_limits(orc):
+    with pytest.raises(orc.InvalidSize):
+        orc.kb_forward(np.zeros(3, dtype=np.uint32))
+    with pytest.raises(orc.InvalidSize):
+        orc.ntt_batch(orc.FIELD_KB, False, 25, 1, np.zeros(1 << 25, dtype=np.uint32))
+
+
 # ----------------------------------------------------------- circle group
 def test_circle_kats(orc):
     # src/field/circle.rs:158-230
@@ -139,15 +200,15 @@ def test_m31_twiddle_layers_match_closed_form(orc):
     assert all(int(a) * int(b) % M31_P == 1 for a, b in zip(tw, itw))
 
 
-@pytest.mark.parametrize("field", [0, 1])
+@pytest.mark.parametrize("field", [0, 1, 2])
 @pytest.mark.parametrize("log_n", [1, 2, 5, 10, 12])
 def test_roundtrip_pins_inverse(orc, field, log_n):
     # forward is pinned by the closed forms above; inverse(forward(x)) == x then
     # fixes inverse uniquely (forward is a bijection).
     n = 1 << log_n
     x = orc.lcg_fill(n * 11111 + 98765, n, field)
-    fwd = orc.m31_forward if field == 0 else orc.bb_forward
-    inv = orc.m31_inverse if field == 0 else orc.bb_inverse
+    fwd = [orc.m31_forward, orc.bb_forward, orc.kb_forward][field]
+    inv = [orc.m31_inverse, orc.bb_inverse, orc.kb_inverse][field]
     y = fwd(x)
     assert not np.array_equal(y, x)
     assert np.array_equal(inv(y), x)
@@ -241,12 +302,12 @@ def test_layouts_are_transposes(orc):
     out_soa = orc.coset_lde_batch(soa, log_n, batch, a)
     out_rm = orc.coset_lde_batch(rm, log_n, batch, a, layout=orc.LAYOUT_ROW_MAJOR)
     assert np.array_equal(out_rm.reshape(N, batch).T.ravel(), out_soa)
-    for field in (0, 1):
+    for field in (0, 1, 2):
         x = orc.lcg_fill(9, n * batch, field)
         y_soa = orc.ntt_batch(field, False, log_n, batch, x)
         y_rm = orc.ntt_batch(field, False, log_n, batch, x.reshape(batch, n).T.copy().ravel(),
                              layout=orc.LAYOUT_ROW_MAJOR)
         assert np.array_equal(y_rm.reshape(n, batch).T.ravel(), y_soa)
-        single = np.concatenate([(orc.m31_forward if field == 0 else orc.bb_forward)(
+        single = np.concatenate([[orc.m31_forward, orc.bb_forward, orc.kb_forward][field](
             x[c * n:(c + 1) * n]) for c in range(batch)])
         assert np.array_equal(single, y_soa)
