@@ -151,7 +151,7 @@ def cpu_baseline_ntt(workload: str) -> dict:
     import oracle as orc
     from sfntt import synth
 
-    field, log_n = (1, 16) if workload == "bb16" else (0, 20)
+    field, log_n = {"bb16": (1, 16), "m31_20": (0, 20), "kb16": (2, 16)}[workload]
     n, cols = 1 << log_n, 8
     x = np.concatenate([synth.field_data(field, 12345 + n, n)] * cols)
     orc.lib()
@@ -177,7 +177,8 @@ def main() -> int:
     ap.add_argument("--cpu-cols", type=int, default=24, help="columns in the CPU baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "uneven", "small"], default="lde")
+    ap.add_argument("--workload", choices=["lde", "bb16", "m31_20", "kb16", "batch20", "uneven", "small"],
+                    default="lde")
     ap.add_argument("--e2e-steps", type=int, default=3,
                     help="timed host-pointer (PCIe-inclusive) LDE calls on pinned buffers (0 = skip)")
     ap.add_argument("--cpu-all-cols", type=int, default=128,
@@ -215,7 +216,7 @@ def main() -> int:
         import workloads  # small-field-metal-ntt_amd/workloads.py
         res = workloads.run(args.workload, ctx, args.steps, args.warmup)
         res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "vs_baseline": None})
-        if args.cpu_cols > 0 and args.workload in ("bb16", "m31_20"):
+        if args.cpu_cols > 0 and args.workload in ("bb16", "m31_20", "kb16"):
             res["cpu_baseline"] = cpu_baseline_ntt(args.workload)
         print(json.dumps(res), flush=True)
         return 0

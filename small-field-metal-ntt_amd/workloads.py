@@ -11,6 +11,10 @@ uneven  512 BabyBear/M31 columns with log_n ~ U{10..22} in one multi-job plan
 small   many small transforms (2^8 and 2^10 points, both fields; the batched
         small-NTT regime of lattice schemes): one batched job, a plan of
         single-column jobs (hipGraph), and single-column launches
+kb16    KoalaBear forward NTT, one column of 2^16, every variant (the bb16
+        shape on the field the reference lists as planned, README.md:20)
+batch20 forward NTT of 256 columns of 2^20 (the LDE's matrix shape) in every
+        field, SoA and row-major, one in-place batched call each
 
 Device time is a hipEvent pair around the launch sequence with inputs
 resident in HBM; "call" time is the host-pointer entry point's wall clock
@@ -27,6 +31,8 @@ import sfntt
 from sfntt import _lib as L
 from sfntt import synth
 
+FIELD_NAMES = {L.FIELD_M31: "M31", L.FIELD_BABYBEAR: "BabyBear", L.FIELD_KOALABEAR: "KoalaBear"}
+FIELD_TAGS = {L.FIELD_M31: "m31", L.FIELD_BABYBEAR: "bb", L.FIELD_KOALABEAR: "kb"}
 VARIANTS = [("v1_ct_dit_r2", L.CT_DIT_R2), ("v2_ct_gs_r2", L.CT_GS_R2), ("v3_stockham_r2", L.STOCKHAM_R2),
             ("v4_ct_gs_r4", L.CT_GS_R4), ("auto", L.VARIANT_AUTO)]
 
@@ -64,20 +70,52 @@ def _single(ctx, field: int, log_n: int, steps: int, warmup: int) -> dict:
 
 
 def run(name: str, ctx, steps: int, warmup: int) -> dict:
-    if name in ("bb16", "m31_20"):
-        field, log_n = (L.FIELD_BABYBEAR, 16) if name == "bb16" else (L.FIELD_M31, 20)
+    if name in ("bb16", "m31_20", "kb16"):
+        field, log_n = {"bb16": (L.FIELD_BABYBEAR, 16), "m31_20": (L.FIELD_M31, 20),
+                        "kb16": (L.FIELD_KOALABEAR, 16)}[name]
         r = _single(ctx, field, log_n, max(steps, 20), max(warmup, 5))
         best = min(r.values(), key=lambda e: e["device_us"])
-        return {"metric": f"{'BabyBear' if field else 'M31'} forward NTT 2^{log_n} (one column), device Melem/s",
+        return {"metric": f"{FIELD_NAMES[field]} forward NTT 2^{log_n} (one column), device Melem/s",
                 "value": best["device_melem_s"], "unit": "Melem/s", "higher_is_better": True,
                 "dtype": "u32", "data": "synthetic (reference LCG, seed 12345+n)",
-                "config": {"workload": f"{'bb' if field else 'm31'}_forward_2^{log_n}_single_column",
+                "config": {"workload": f"{FIELD_TAGS[field]}_forward_2^{log_n}_single_column",
                            "log_n": log_n, "batch": 1}, "variants": r}
+    if name == "batch20":
+        return _batch20(ctx, steps, warmup)
     if name == "uneven":
         return _uneven(ctx, steps, warmup)
     if name == "small":
         return _small(ctx, steps, warmup)
     raise ValueError(name)
+
+
+def _batch20(ctx, steps, warmup):
+    lib = ctx._lib
+    log_n, cols = 20, 256
+    total = cols << log_n
+    buf = sfntt.DeviceBuffer(ctx, total)
+    ns = ctypes.c_uint64()
+    modes = {}
+    for field in (L.FIELD_KOALABEAR, L.FIELD_BABYBEAR, L.FIELD_M31):
+        buf.upload(synth.field_data(field, 42 + log_n, total))
+        for lay, lname in ((L.LAYOUT_SOA, "soa"), (L.LAYOUT_ROW_MAJOR, "rowmajor")):
+            dev = []
+            for k in range(warmup + steps):
+                ctx.check(lib.sfn_ntt_device(ctx.ptr, field, L.VARIANT_AUTO, L.FORWARD, log_n, cols, lay, buf.ptr,
+                                             buf.ptr, None))
+                ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+                if k >= warmup:
+                    dev.append(ns.value / 1e3)
+            d = float(np.median(dev))
+            modes[f"{FIELD_TAGS[field]}_{lname}"] = {"device_us": round(d, 1),
+                                                    "gelem_s": round(total / d / 1e3, 2),
+                                                    "hbm_gb_s_min": round(2 * 4 * total / d / 1e3, 1)}
+    kb = modes["kb_rowmajor"]
+    return {"metric": "KoalaBear forward NTT, 256 columns of 2^20 (row-major), device Gelem/s",
+            "value": kb["gelem_s"], "unit": "Gelem/s", "higher_is_better": True, "dtype": "u32",
+            "data": "synthetic (reference LCG, seed 42+log_n)",
+            "config": {"workload": "forward_ntt_2^20x256_all_fields", "log_n": log_n, "batch": cols},
+            "modes": modes}
 
 
 def _timed(ctx, fn, steps, warmup):
