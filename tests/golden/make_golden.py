@@ -5,16 +5,17 @@
 fixtures.npz   small input/output vectors: BabyBear and M31 forward/inverse
                at 2^4 and 2^9, coset LDEs (8,4,1) and (8,4,2) at the
                reference shift and at shift 31 (src/lde/mod.rs:194-271 cases)
-digests.json   SHA-256 of the full outputs at the benchmark configurations,
-               over the little-endian u32 bytes in the stated layout:
-                 config1  BabyBear forward 2^16, seed 12345 + 2^16
-                 config2  M31 forward 2^20, seed 12345 + 2^20
-                 config3  coset LDE 2^20 x 256, 2x, seed 42 + 20 (inputs
-                          generated column by column): row-major with shift 31,
-                          and SoA with the reference shift
-                 config4  512 uneven columns (SplitMix64 seed 0x5EED0004:
-                          field = draw1 & 1, log_n = 10 + draw2 % 13, LCG seed
-                          = draw3), forward, outputs concatenated in order
+digests.json   SHA-256 of the full outputs of the benchmark workloads, over
+               the little-endian u32 bytes in the stated layout:
+                 bb_fwd_2e16     BabyBear forward 2^16, seed 12345 + 2^16
+                 m31_fwd_2e20    M31 forward 2^20, seed 12345 + 2^20
+                 lde_2e20x256_*  coset LDE 2^20 x 256, 2x, seed 42 + 20 (inputs
+                                 generated column by column): row-major with
+                                 shift 31, and SoA with the reference shift
+                 uneven_512_columns_forward
+                                 512 uneven columns (SplitMix64 seed 0x5EED0004:
+                                 field = draw1 & 1, log_n = 10 + draw2 % 13, LCG
+                                 seed = draw3), forward, outputs concatenated
 Inputs come from the oracle's restatement of the reference LCG; every output is
 the oracle's.  The files are data only (numpy arrays without pickling, JSON).
 """
@@ -70,20 +71,20 @@ def digests() -> dict:
     d["lde_12_8_1_ref"] = sha(orc.coset_lde_batch(x, 12, 8, 1))
     d["lde_12_8_1_g31"] = sha(orc.coset_lde_batch(x, 12, 8, 1, orc.bb_to_monty(31)))
     n = 1 << 16
-    d["config1_bb_fwd_2e16"] = sha(orc.bb_forward(orc.lcg_fill(12345 + n, n, BB)))
+    d["bb_fwd_2e16"] = sha(orc.bb_forward(orc.lcg_fill(12345 + n, n, BB)))
     n = 1 << 20
-    d["config2_m31_fwd_2e20"] = sha(orc.m31_forward(orc.lcg_fill(12345 + n, n, M31)))
+    d["m31_fwd_2e20"] = sha(orc.m31_forward(orc.lcg_fill(12345 + n, n, M31)))
     log_n, B, a = 20, 256, 1
     x = orc.lcg_fill(42 + log_n, B << log_n, BB)  # column by column (SoA order)
     y = orc.coset_lde_batch(x, log_n, B, a, 0)
-    d["config3_lde_2e20x256_2x_soa_refshift"] = sha(y)
+    d["lde_2e20x256_2x_soa_refshift"] = sha(y)
     y = orc.coset_lde_batch(x, log_n, B, a, orc.bb_to_monty(31))  # SoA result, shift 31
-    d["config3_lde_2e20x256_2x_rowmajor_shift31"] = sha(y.reshape(B, 2 << log_n).T)
+    d["lde_2e20x256_2x_rowmajor_shift31"] = sha(y.reshape(B, 2 << log_n).T)
     h = hashlib.sha256()
     for field, ln, seed in uneven_shapes():
         x = orc.lcg_fill(seed, 1 << ln, field)
         h.update(np.ascontiguousarray(orc.ntt_batch(field, False, ln, 1, x), dtype="<u4").tobytes())
-    d["config4_uneven_512_forward"] = h.hexdigest()
+    d["uneven_512_columns_forward"] = h.hexdigest()
     return d
 
 

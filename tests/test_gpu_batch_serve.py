@@ -23,7 +23,7 @@ def ctx():
 
 
 def uneven_jobs(seed, count, lo, hi):
-    """Config-4 style draw: field ~ Bernoulli(1/2), log_n ~ U{lo..hi}, per-column LCG seed."""
+    """Uneven-columns draw: field ~ Bernoulli(1/2), log_n ~ U{lo..hi}, per-column LCG seed."""
     st = orc.splitmix64_stream(seed)
     jobs = []
     for _ in range(count):
@@ -145,7 +145,7 @@ def test_graph_replays_each_match(ctx):
     ctx._lib.sfn_plan_destroy(plan)
 
 
-def test_config4_full_size_plan(ctx):
+def test_uneven_512_columns_full_size_plan(ctx):
     # 512 columns, log_n ~ U{10..22}, both fields (the workload as stated)
     specs = uneven_jobs(0x5EED0004, 512, 10, 22)
     inputs, outs, launches = run_plan(ctx, specs, 1)
@@ -185,7 +185,7 @@ def make_server(ctx, **kw):
 
 
 def mixed_requests(seed, count):
-    """Config-5 style mix (scaled down): 60% BabyBear LDE, 40% M31 NTT."""
+    """Mixed serving trace (scaled down): 60% BabyBear LDE, 40% M31 NTT."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):

@@ -1,8 +1,8 @@
 """GPU outputs against the committed golden fixtures and the SHA-256 digests of
-the full benchmark configurations (tests/golden/, written from the oracle by
-tests/golden/make_golden.py).  Full sizes: config 1 (BabyBear 2^16), config 2
-(M31 2^20, all variants), config 3 (LDE 2^20 x 256 2x, row-major shift 31 and
-SoA reference shift), config 4 (512 uneven columns through a graph plan)."""
+the full benchmark workloads (tests/golden/, written from the oracle by
+tests/golden/make_golden.py).  Full sizes: BabyBear forward 2^16 and M31
+forward 2^20 (all variants), the coset LDE 2^20 x 256 2x (row-major shift 31
+and SoA reference shift), 512 uneven columns through a graph plan."""
 import ctypes
 import hashlib
 import json
@@ -58,18 +58,18 @@ def test_small_fixtures(ctx, fx):
         assert np.array_equal(y, fx[f"lde_{log_n}_{batch}_{a}_g31_out"])
 
 
-def test_config1_and_2_digests(ctx, digests):
+def test_single_column_digests(ctx, digests):
     n = 1 << 16
     x = orc.lcg_fill(12345 + n, n, BB)
     for v in (sfntt.CT_DIT_R2, sfntt.CT_GS_R2, sfntt.STOCKHAM_R2, sfntt.CT_GS_R4, sfntt.VARIANT_AUTO):
         y, _ = ctx.ntt(BB, v, sfntt.FORWARD, 16, 1, x)
-        assert sha(y) == digests["config1_bb_fwd_2e16"], f"config 1 v{v}"
+        assert sha(y) == digests["bb_fwd_2e16"], f"BabyBear 2^16 v{v}"
     n = 1 << 20
     x = orc.lcg_fill(12345 + n, n, M31)
     for v in (sfntt.CT_DIT_R2, sfntt.CT_GS_R2, sfntt.STOCKHAM_R2, sfntt.CT_GS_R4, sfntt.VARIANT_AUTO):
-        print(f"[golden] config2 v{v}", flush=True)
+        print(f"[golden] m31 2^20 v{v}", flush=True)
         y, _ = ctx.ntt(M31, v, sfntt.FORWARD, 20, 1, x)
-        assert sha(y) == digests["config2_m31_fwd_2e20"], f"config 2 v{v}"
+        assert sha(y) == digests["m31_fwd_2e20"], f"M31 2^20 v{v}"
     x = orc.lcg_fill(42, 8 << 12, BB)
     y, _ = ctx.coset_lde(x, 12, 8, 1)
     assert sha(y) == digests["lde_12_8_1_ref"]
@@ -77,30 +77,30 @@ def test_config1_and_2_digests(ctx, digests):
     assert sha(y) == digests["lde_12_8_1_g31"]
 
 
-def test_config3_full_lde_digests(ctx, digests):
+def test_lde_2e20x256_digests(ctx, digests):
     log_n, B, a = 20, 256, 1
     n, N = 1 << log_n, 2 << log_n
     x = orc.lcg_fill(42 + log_n, B << log_n, BB)  # generated column by column
     d_in, d_out = sfntt.DeviceBuffer(ctx, n * B), sfntt.DeviceBuffer(ctx, N * B)
     try:
-        print("[golden] config3 row-major shift 31", flush=True)
+        print("[golden] lde 2^20 x 256 row-major shift 31", flush=True)
         d_in.upload(x.reshape(B, n).T.copy().ravel())
         ctx.check(ctx._lib.sfn_coset_lde_batch_device(ctx.ptr, log_n, B, a, orc.bb_to_monty(31),
                                                       L.LAYOUT_ROW_MAJOR, d_in.ptr, d_out.ptr, None))
         ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
-        assert sha(d_out.download()) == digests["config3_lde_2e20x256_2x_rowmajor_shift31"]
-        print("[golden] config3 SoA reference shift", flush=True)
+        assert sha(d_out.download()) == digests["lde_2e20x256_2x_rowmajor_shift31"]
+        print("[golden] lde 2^20 x 256 SoA reference shift", flush=True)
         d_in.upload(x)
         ctx.check(ctx._lib.sfn_coset_lde_batch_device(ctx.ptr, log_n, B, a, 0, L.LAYOUT_SOA, d_in.ptr,
                                                       d_out.ptr, None))
         ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
-        assert sha(d_out.download()) == digests["config3_lde_2e20x256_2x_soa_refshift"]
+        assert sha(d_out.download()) == digests["lde_2e20x256_2x_soa_refshift"]
     finally:
         d_in.free()
         d_out.free()
 
 
-def test_config4_uneven_plan_digest(ctx, digests):
+def test_uneven_512_columns_plan_digest(ctx, digests):
     it = orc.splitmix64_stream(0x5EED0004)
     shapes = []
     for _ in range(512):
@@ -116,13 +116,13 @@ def test_config4_uneven_plan_digest(ctx, digests):
     plan = ctypes.c_void_p()
     ctx.check(ctx._lib.sfn_plan_create(ctx.ptr, arr, len(descs), 1, ctypes.byref(plan)))
     try:
-        print("[golden] config4 plan (graph)", flush=True)
+        print("[golden] uneven 512 columns plan (graph)", flush=True)
         ctx.check(ctx._lib.sfn_plan_launch(plan, None))
         ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
         h = hashlib.sha256()
         for b in bufs:
             h.update(np.ascontiguousarray(b.download(), dtype="<u4").tobytes())
-        assert h.hexdigest() == digests["config4_uneven_512_forward"]
+        assert h.hexdigest() == digests["uneven_512_columns_forward"]
     finally:
         ctx._lib.sfn_plan_destroy(plan)
         for b in bufs:

@@ -61,7 +61,7 @@ def test_single_column_sizes(ctx, field, variant):
         assert_same(z, ref_inv(field, x), f"inverse field={field} variant={variant} log_n={log_n}")
 
 
-def test_config1_babybear_2e16_all_variants(ctx):
+def test_babybear_forward_2e16_all_variants(ctx):
     # BabyBear forward 2^16, natural-order input, seed 12345 + 2^16 (benches/ntt_benchmark.rs:654)
     n = 1 << 16
     x = orc.lcg_fill(12345 + n, n, BB)
@@ -72,14 +72,14 @@ def test_config1_babybear_2e16_all_variants(ctx):
         assert_same(got, exp, b.name())
 
 
-def test_config2_m31_2e20_all_variants(ctx):
+def test_m31_forward_2e20_all_variants(ctx):
     # M31 circle NTT 2^20, seed 12345 + 2^20 (benches/ntt_benchmark.rs:206)
     n = 1 << 20
     x = orc.lcg_fill(12345 + n, n, M31)
     exp = orc.m31_forward(x)
     for cls in api.M31_VARIANTS:
         b = cls(ctx=ctx)
-        print(f"[parity] config2 {b.name()}", flush=True)
+        print(f"[parity] m31 2^20 {b.name()}", flush=True)
         got, _ = b.forward_ntt_gpu(x, 20)
         assert_same(got, exp, b.name())
         back, _ = b.inverse_ntt_gpu(got, 20)

@@ -40,9 +40,11 @@ def ctx():
     return sfntt.HipContext(0)
 
 
-CASES_NTT = [(M31, 12, 1), (M31, 17, 3), (M31, 20, 1), (BB, 11, 5), (BB, 16, 4), (BB, 20, 2), (KB, 13, 3),
-             (KB, 20, 2)]
-CASES_LDE = [(10, 24, 1), (14, 16, 2), (16, 8, 1)]
+# (2^20 cases with batches of 8 and 16: row-major 2^10-point passes over
+# column-fastest tiles, anchored on the oracle below)
+CASES_NTT = [(M31, 12, 1), (M31, 17, 3), (M31, 20, 1), (M31, 20, 16), (BB, 11, 5), (BB, 16, 4), (BB, 20, 2),
+             (BB, 20, 8), (KB, 13, 3), (KB, 20, 8)]
+CASES_LDE = [(10, 24, 1), (14, 16, 2), (16, 8, 1), (20, 8, 1)]
 
 
 def _run_all(c):
@@ -69,7 +71,13 @@ def reference_outputs(ctx):
     # anchor the default path on the oracle for a few of the cases
     x = orc.lcg_fill(1000 + 16, 4 << 16, BB)
     exp = orc.ntt_batch(BB, False, 16, 4, x)
-    assert np.array_equal(outs[4 * 4], exp), "default path differs from the oracle"
+    assert np.array_equal(outs[4 * CASES_NTT.index((BB, 16, 4))], exp), "default path differs from the oracle"
+    for i, (field, log_n, batch) in enumerate(CASES_NTT):
+        if log_n == 20 and batch >= 8:  # row-major outputs
+            x = orc.lcg_fill(1000 + log_n, batch << log_n, field)
+            for d in (0, 1):
+                exp = orc.ntt_batch(field, d == 1, log_n, batch, x, layout=orc.LAYOUT_ROW_MAJOR)
+                assert np.array_equal(outs[4 * i + 2 + d], exp), f"row-major field={field} batch={batch} dir={d}"
     for i, (field, log_n, batch) in enumerate(CASES_NTT):
         if field == KB:  # KoalaBear: forward and inverse, SoA
             x = orc.lcg_fill(1000 + log_n, batch << log_n, KB)
@@ -77,6 +85,9 @@ def reference_outputs(ctx):
             assert np.array_equal(outs[4 * i + 1], orc.ntt_batch(KB, True, log_n, batch, x)), "KoalaBear inverse"
     x = orc.lcg_fill(42 + 14, 16 << 14, BB)
     assert np.array_equal(outs[4 * len(CASES_NTT) + 2], orc.coset_lde_batch(x, 14, 16, 2)), "default LDE differs"
+    x = orc.lcg_fill(42 + 20, 8 << 20, BB)
+    exp = orc.coset_lde_batch(x, 20, 8, 1, orc.bb_to_monty(31), orc.LAYOUT_ROW_MAJOR)
+    assert np.array_equal(outs[4 * len(CASES_NTT) + 2 * 3 + 1], exp), "row-major 2^20 x 8 LDE differs"
     return outs
 
 
