@@ -26,6 +26,9 @@ sys.path.insert(0, os.path.join(ROOT, "small-field-metal-ntt_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "lde_2e20x256_hbm_counters.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "lde_2e20x256_valu_counters.txt")
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles per SIMD, 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def kernel_bytes(label: str, log_n: int, batch: int, added_bits: int) -> int:
@@ -47,6 +50,19 @@ def traffic_key(label: str) -> str:
     if head == "k_fast" or head == "k_pass":
         return head + ("_inv" if ",inv," in label else "_fwd")
     return head
+
+
+def read_valu(key: str):
+    """VALU (wave64) instructions per launch from the committed rocprofv3 --pmc summary, if present."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    for line in open(VALU_FILE):
+        parts = line.split("#")[0].split()
+        if parts and parts[0] == key:
+            kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
+            if "valu_insts" in kv:
+                return float(kv["valu_insts"])
+    return None
 
 
 def read_traffic(key: str):
@@ -264,6 +280,7 @@ def main() -> int:
     dom_bytes = kernel_bytes(dom, log_n, B, a)
     achieved = dom_bytes / (kernels[dom] * 1e-6) / 1e9
     traffic = read_traffic(traffic_key(dom))
+    valu_insts = read_valu(traffic_key(dom))
     lde_bytes = 4 * B * (n + N)
     pipe_gbs = lde_bytes / (ms_per_step * 1e-3) / 1e9
 
@@ -309,6 +326,13 @@ def main() -> int:
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+        },
+        # the same kernel against the VALU issue peak (the bound it is closest to)
+        "roofline_valu": None if valu_insts is None else {
+            "bound": "valu", "kernel": dom, "insts_per_launch": valu_insts,
+            "achieved": round(valu_insts * 64 / (kernels[dom] * 1e-6) / 1e12, 2),
+            "peak": round(VALU_PEAK_WAVE_INSTS * 64 / 1e12, 2), "unit": "Tlane-op/s",
+            "frac": round(valu_insts / (kernels[dom] * 1e-6) / VALU_PEAK_WAVE_INSTS, 4),
         },
         "pipeline": {
             "algorithmic_bytes": lde_bytes,
