@@ -316,24 +316,28 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
 // j << (ROOT_LOG - t - 1) whatever the sub-transform size, so one table of
 // 2^11 entries serves every tile size.  The caller fills it before the call
 // (the barrier after the lane set-up publishes it).
-template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG>
+// LW >= 0: the launch's lanes per workgroup are 2^LW (a.log_w == LW), known at
+// compile time, so every LDS slot index of the rounds folds to an immediate
+// offset from one per-item base.
+template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG, int LW = -1>
 __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, LaneInfo *lanes, const uint2 *root,
                                           uint32_t *tile) {
   static_assert(ROOT_LOG >= LOG_T, "root table too small");
+  const uint32_t log_w = LW >= 0 ? (uint32_t)LW : a.log_w;
   RoundIO io;
   setup<LOG_T>(a, lanes, io, tile_id);
   __syncthreads();
   if constexpr (F::MONTY) {
     RootTw<ROOT_LOG, DIR> tw;
     tw.tab = root;
-    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, a.log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
   } else {
     const auto tw = make_tw<F, DIR, LOG_T>(a, nullptr);
-    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, a.log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
   }
 }
 
-template <class F, int DIR, int LOG_T, bool CONTIG>
+template <class F, int DIR, int LOG_T, bool CONTIG, int LW = -1>
 __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   uint32_t *tile = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(root) +
                                                 (F::MONTY ? ((size_t)8 << LOG_T) : 0));
   if constexpr (F::MONTY) load_root<LOG_T>(a, root);
-  fast_tile<F, DIR, LOG_T, CONTIG, LOG_T>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
+  fast_tile<F, DIR, LOG_T, CONTIG, LOG_T, LW>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
 }
 
 // ---- coset LDE middle pass ----
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
 // the twist, point-factor and post tables it needs are T words each; they are
 // staged in LDS ([tables: 2T words] between the root table and the tile)
 // instead of being read per lane from global memory.
-template <int LOG_T, bool LT>
+template <int LOG_T, bool LT, int LW = -1>
 __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   using F = BabyBear;
   using S = Sched<LOG_T, KR>;
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   constexpr int R = 1 << K;
   constexpr uint32_t T = 1u << LOG_T;
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t log_w = a.log_w;
+  const uint32_t log_w = LW >= 0 ? (uint32_t)LW : a.log_w;  // a.log_w == LW when LW >= 0
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
   unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
   uint32_t *tabs = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
@@ -455,12 +459,20 @@ static hipError_t launch_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - KR);
   if (lds > kMaxLdsBytes || threads > 512 || tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
   static bool configured = false;
+  // forward Montgomery-field 2^10-point passes over 16 adjacent lanes (the
+  // coset LDE's last pass, batched forward transforms): lane count fixed at
+  // compile time.  Measured: 3.5 % faster there; the inverse and M31 passes
+  // ran 6-7 % slower with it, so they keep the runtime lane count.
+  constexpr bool kLw4 = CONTIG && LOG_T == 10 && F::MONTY && DIR == 0;
   auto kern = k_fast<F, DIR, LOG_T, CONTIG>;
+  auto kern4 = k_fast<F, DIR, LOG_T, CONTIG, kLw4 ? 4 : -1>;
   if (!configured) {
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+    (void)hipFuncSetAttribute((const void *)kern4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     configured = true;
   }
-  hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  if (kLw4 && a.log_w == 4) hipLaunchKernelGGL(kern4, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  else hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   return hipGetLastError();
 }
 
@@ -523,10 +535,16 @@ static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t l
   if (!configured) {
     (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kMaxLdsBytes);
+    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 3>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     configured = true;
   }
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
-  hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  // eight lanes (the default middle-pass width): lane count fixed at compile time
+  if (LOG_T == 10 && a.log_w == 3)
+    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, 3>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  else
+    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   return hipGetLastError();
 }
 
