@@ -458,7 +458,6 @@ static hipError_t launch_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
   const size_t lds = lds_bytes(F::ID, LOG_T, a.log_w);
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - KR);
   if (lds > kMaxLdsBytes || threads > 512 || tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
-  static bool configured = false;
   // forward Montgomery-field 2^10-point passes over 16 adjacent lanes (the
   // coset LDE's last pass, batched forward transforms): lane count fixed at
   // compile time.  Measured: 3.5 % faster there; the inverse and M31 passes
@@ -466,11 +465,14 @@ static hipError_t launch_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
   constexpr bool kLw4 = CONTIG && LOG_T == 10 && F::MONTY && DIR == 0;
   auto kern = k_fast<F, DIR, LOG_T, CONTIG>;
   auto kern4 = k_fast<F, DIR, LOG_T, CONTIG, kLw4 ? 4 : -1>;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     (void)hipFuncSetAttribute((const void *)kern4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   if (kLw4 && a.log_w == 4) hipLaunchKernelGGL(kern4, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   else hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   return hipGetLastError();
@@ -531,14 +533,16 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
 
 template <int LOG_T, bool LT>
 static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t lds, hipStream_t s) {
-  static bool configured = false;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kMaxLdsBytes);
     (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 3>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
   // eight lanes (the default middle-pass width): lane count fixed at compile time
   if (LOG_T == 10 && a.log_w == 3)
@@ -648,12 +652,14 @@ hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t tota
                              uint32_t xs, const void *bb_root, uint32_t lanes_bytes, hipStream_t s) {
   if (total_tiles == 0) return hipSuccess;
   if (lds > kMaxLdsBytes || grid == 0 || xs > 3) return hipErrorInvalidConfiguration;
-  static bool configured = false;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)fast::k_multi_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   hipLaunchKernelGGL(fast::k_multi_fast, dim3(grid), dim3(kMultiFastThreads), lds, s, d_segs, nseg, total_tiles, xs,
                      reinterpret_cast<const uint2 *>(bb_root), lanes_bytes);
   return hipGetLastError();

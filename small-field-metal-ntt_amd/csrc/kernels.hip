@@ -370,11 +370,13 @@ static hipError_t launch_pass_t(const PassArgs &a, uint64_t tiles, hipStream_t s
   const size_t lds = pass_lds_bytes(F::ID, LOG_T, a.log_w);
   if (lds > kMaxLdsBytes) return hipErrorInvalidConfiguration;
   auto kern = k_pass<F, DIR, LOG_T, LOGR>;
-  static bool configured = false;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
@@ -461,11 +463,13 @@ static hipError_t launch_mid_t(const PassArgs &a, uint64_t tiles, hipStream_t s)
   const size_t lds = lde_mid_lds_bytes(LOG_T, a.log_w);
   if (lds > kMaxLdsBytes) return hipErrorInvalidConfiguration;
   auto kern = k_lde_mid<LOG_T, LOGR>;
-  static bool configured = false;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   const uint64_t threads = (uint64_t)(1u << a.log_w) << (LOG_T - K);
   if (threads > 1024 || threads < 1) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
@@ -578,11 +582,13 @@ hipError_t launch_multi(const SegDesc *d_segs, uint32_t nseg, uint64_t total_til
                         hipStream_t s) {
   if (total_tiles == 0) return hipSuccess;
   if (lds > kMaxLdsBytes || grid == 0) return hipErrorInvalidConfiguration;
-  static bool configured = false;
-  if (!configured) {
+  // one-time attribute set-up; a function-local static initialises once,
+  // also when several host threads launch at the same time
+  static const bool configured = [&] {
     (void)hipFuncSetAttribute((const void *)k_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    configured = true;
-  }
+    return true;
+  }();
+  (void)configured;
   hipLaunchKernelGGL(k_multi, dim3(grid), dim3(NT), lds, s, d_segs, nseg, total_tiles);
   return hipGetLastError();
 }

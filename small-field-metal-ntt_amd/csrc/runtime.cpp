@@ -80,7 +80,9 @@ void Context::release_lde_scratch(hipStream_t s) {
 sfn_status Context::ensure(DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes && b.ptr) return SFN_OK;
   if (b.ptr) {
-    (void)hipStreamSynchronize(stream);
+    // kernels on any stream may still read the old buffer (tables are shared
+    // by every stream of the context); growth is rare, so wait for the device
+    (void)hipDeviceSynchronize();
     (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
