@@ -115,3 +115,71 @@ def test_autotuned_schedules_give_same_bits(ctx, reference_outputs):
     for i, (g, r) in enumerate(zip(got, reference_outputs)):
         assert np.array_equal(g, r), f"case {i} differs under autotuned schedules"
     c.close()
+
+
+def autotune_candidates(log_n, lde):
+    """The schedules the autotuner times for one shape (batch.cpp, sfn_autotune):
+    (largest tile, register radix, lanes per workgroup, middle-pass lanes)."""
+    for mlt in range(6, 12):
+        if (log_n + mlt - 1) // mlt > 3 or (lde and log_n > 2 * mlt):
+            continue
+        for logr in (3, 4, 5):
+            for lw in (2, 3, 4, 5):
+                for mlw in ((2, 3, 4, 5) if lde else (lw,)):
+                    if mlt + max(lw, mlw) <= 15:
+                        yield mlt, logr, lw, mlw
+
+
+def _force_schedule(c, key_field, log_n, batch, layout, sched):
+    mlt, logr, lw, mlw = sched
+    js = (f'[{{"field":{key_field},"log_n":{log_n},"batch":{batch},"layout":{layout},'
+          f'"max_log_t":{mlt},"logr":{logr},"log_w":{lw},"mid_log_w":{mlw}}}]')
+    c.check(c._lib.sfn_tuning_load(c.ptr, js.encode()))
+
+
+NOT_LAUNCHABLE = (sfntt.api.Unsupported, sfntt.api.InvalidArgument, sfntt.api.InvalidSize)
+
+
+@pytest.mark.parametrize("field,log_n,batch,layout", [
+    (BB, 16, 4, sfntt.LAYOUT_SOA), (BB, 16, 4, sfntt.LAYOUT_ROW_MAJOR), (M31, 17, 3, sfntt.LAYOUT_SOA),
+    (KB, 13, 3, sfntt.LAYOUT_ROW_MAJOR)])
+def test_every_autotuner_schedule_matches_oracle(field, log_n, batch, layout):
+    """Each candidate the autotuner may pick gives the oracle's bits, forward and inverse
+    (the autotuner itself only times them)."""
+    c = sfntt.HipContext(0)
+    x = orc.lcg_fill(1000 + log_n, batch << log_n, field)
+    exp = [orc.ntt_batch(field, d == 1, log_n, batch, x, layout=layout) for d in (0, 1)]
+    ran = 0
+    for sched in autotune_candidates(log_n, False):
+        print(f"[sched] field={field} log_n={log_n} batch={batch} layout={layout} schedule={sched}", flush=True)
+        _force_schedule(c, field, log_n, batch, layout, sched)
+        for d in (sfntt.FORWARD, sfntt.INVERSE):
+            try:
+                y, _ = c.ntt(field, sfntt.VARIANT_AUTO, d, log_n, batch, x, layout)
+            except NOT_LAUNCHABLE:
+                continue
+            assert np.array_equal(y, exp[d]), f"schedule {sched} dir {d}"
+            ran += 1
+    assert ran >= 20
+    c.close()
+
+
+@pytest.mark.parametrize("log_n,batch,layout", [(12, 6, sfntt.LAYOUT_SOA), (13, 8, sfntt.LAYOUT_ROW_MAJOR)])
+def test_every_autotuner_lde_schedule_matches_oracle(log_n, batch, layout):
+    """Each LDE candidate (outer and middle-pass lanes tuned apart) gives the oracle's bits."""
+    c = sfntt.HipContext(0)
+    x = orc.lcg_fill(42 + log_n, batch << log_n, BB)
+    shift = 0 if layout == sfntt.LAYOUT_SOA else orc.bb_to_monty(31)
+    exp = orc.coset_lde_batch(x, log_n, batch, 1, shift, layout)
+    ran = 0
+    for sched in autotune_candidates(log_n, True):
+        print(f"[sched] lde log_n={log_n} batch={batch} layout={layout} schedule={sched}", flush=True)
+        _force_schedule(c, L.TUNE_LDE, log_n, batch, layout, sched)
+        try:
+            y, _ = c.coset_lde(x, log_n, batch, 1, shift, layout)
+        except NOT_LAUNCHABLE:
+            continue
+        assert np.array_equal(y, exp), f"schedule {sched}"
+        ran += 1
+    assert ran >= 20
+    c.close()
