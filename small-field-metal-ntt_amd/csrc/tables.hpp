@@ -1,6 +1,8 @@
 // Host-side generation of twiddle, twist and coset-factor tables.
 // All arithmetic is exact modular arithmetic, so any evaluation order gives
-// the same values as the reference's generators.
+// the same values as the reference's generators.  Large tables are filled in
+// chunks on up to 16 host threads; each chunk starts from a power (or circle
+// point) computed directly, so the result does not depend on the split.
 #pragma once
 #include <stdint.h>
 
@@ -57,13 +59,16 @@ struct Pair {
 };
 inline Pair shoup(uint32_t w, uint32_t p) { return Pair{w, (uint32_t)(((uint64_t)w << 32) / p)}; }
 
+// Low `bits` bits of v in reverse order (bits <= 64; 0 -> 0).
 inline uint64_t bitrev(uint64_t v, uint32_t bits) {
-  uint64_t r = 0;
-  for (uint32_t i = 0; i < bits; ++i) {
-    r = (r << 1) | (v & 1);
-    v >>= 1;
-  }
-  return r;
+  if (bits == 0) return 0;
+  v = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+  v = ((v >> 2) & 0x3333333333333333ull) | ((v & 0x3333333333333333ull) << 2);
+  v = ((v >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((v & 0x0F0F0F0F0F0F0F0Full) << 4);
+  v = ((v >> 8) & 0x00FF00FF00FF00FFull) | ((v & 0x00FF00FF00FF00FFull) << 8);
+  v = ((v >> 16) & 0x0000FFFF0000FFFFull) | ((v & 0x0000FFFF0000FFFFull) << 16);
+  v = (v >> 32) | (v << 32);
+  return v >> (64 - bits);
 }
 
 // Position table (BabyBear: src/field/babybear.rs:83-97; inverse:

@@ -249,6 +249,61 @@ def test_large_single_transforms(ctx, field):
             assert int(y[0]) == int(x.astype(np.uint64).sum() % p), f"X_0 at 2^{log_n}"
 
 
+@pytest.mark.parametrize("field,log_n", [(BB, 27), (KB, 24), (M31, 30)], ids=["bb_2e27", "kb_2e24", "m31_2e30"])
+def test_maximum_sizes(ctx, field, log_n):
+    # The largest single transform validation accepts (the two-adicity of
+    # BabyBear / KoalaBear; 2^30 for M31, as the reference's GPU paths):
+    # forward against the oracle, then the inverse round trip.  M31 2^30 (4 GiB
+    # per array, where the oracle would take minutes): the forward of a sparse
+    # input against the circle transform's closed form at sampled outputs,
+    # out[i] = sum_k a[k] * prod_{bit l of k} phi_l(P_i).
+    n = 1 << log_n
+    print(f"[parity] maximum size field={field} log_n={log_n}", flush=True)
+    if field == M31:
+        import closed_forms as cf
+        M31P = cf.M31_P
+        ks = [0, 1, 2, 3, 1 << 17, (1 << 29) + 5, n - 1]
+        vals = [7, 123456789, 5, M31P - 1, 999, 31337, 2]
+        a = np.zeros(n, np.uint32)
+        a[ks] = vals
+        y, _ = ctx.ntt(M31, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, 1, a)
+        del a
+        g2n, gn = cf.c_gen(log_n + 1), cf.c_gen(log_n)
+        rng = np.random.default_rng(30)
+        for i in [0, 1, n // 2, n - 1] + [int(v) for v in rng.integers(0, n, 60)]:
+            q = cf.c_add(g2n, cf.c_mul(gn, cf.bitrev(i, log_n)))
+            phi = []
+            for l in range(log_n):
+                phi.append(q[1] if l == log_n - 1 else q[0])
+                q = cf.c_add(q, q)
+            exp = 0
+            for k, v in zip(ks, vals):
+                b = v
+                for l in range(log_n):
+                    if (k >> l) & 1:
+                        b = b * phi[l] % M31P
+                exp = (exp + b) % M31P
+            assert int(y[i]) == exp, f"M31 2^30 forward of a sparse input, output {i}"
+        del y
+    x = data(field, n, 2700 + log_n)
+    y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.FORWARD, log_n, 1, x)
+    if field != M31:
+        assert_same(y, ref_fwd(field, x), f"forward 2^{log_n} field={field}")
+    z, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, sfntt.INVERSE, log_n, 1, y)
+    assert_same(z, x, f"round trip 2^{log_n} field={field}")
+
+
+@pytest.mark.parametrize("log_n,added_bits,layout", [(25, 1, sfntt.LAYOUT_SOA), (24, 2, sfntt.LAYOUT_ROW_MAJOR)],
+                         ids=["2e25_2x_soa", "2e24_4x_rowmajor"])
+def test_lde_maximum_sizes(ctx, log_n, added_bits, layout):
+    # log_n + added_bits + 1 = 27, the largest LDE validation accepts, against the oracle
+    x = data(BB, 1 << log_n, 2500 + log_n)
+    shift = 0 if layout == sfntt.LAYOUT_SOA else orc.bb_to_monty(31)
+    print(f"[parity] maximum LDE log_n={log_n} added_bits={added_bits} layout={layout}", flush=True)
+    y, _ = ctx.coset_lde(x, log_n, 1, added_bits, shift, layout)
+    assert_same(y, orc.coset_lde_batch(x, log_n, 1, added_bits, shift, layout), f"LDE 2^{log_n} x{1 << added_bits}")
+
+
 @pytest.mark.parametrize("log_n,batch,added_bits", [(1, 3, 1), (6, 5, 2), (12, 8, 1), (15, 24, 1), (20, 2, 1),
                                                     (21, 2, 2)])
 @pytest.mark.parametrize("shift", [0, 31], ids=["ref_shift", "generator"])
