@@ -155,8 +155,16 @@ sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t b
                                         const uint32_t *d_in, uint32_t *d_out, void *stream);
 
 /* Device time of the last launch sequence issued by this context on its own
- * stream (hipEvent pair), in nanoseconds; waits for it to finish. */
+ * stream (hipEvent pair), in nanoseconds; waits for it to finish.  Host-pointer
+ * calls are always timed; *_device calls only while sfn_call_timing is on
+ * (otherwise SFN_ERR_INVALID_ARGUMENT). */
 sfn_status sfn_last_device_ns(sfn_ctx *ctx, uint64_t *ns);
+
+/* Per-call timing of the *_device entry points (no reference counterpart; the
+ * reference times its calls on the host).  Off by default: the event pair
+ * around each call costs ~8-10 us of device time, which dominates launch-bound
+ * calls (one 2^8-point column: 17 us per call with it, 8 us without). */
+sfn_status sfn_call_timing(sfn_ctx *ctx, int enable);
 
 /* Per-kernel timing (no reference counterpart).  While enabled, every kernel
  * the context launches on its own stream is bracketed by a hipEvent pair;

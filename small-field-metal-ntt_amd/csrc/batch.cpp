@@ -189,6 +189,7 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
     }
   }
   c.tuned[sfn::TuneKey{lde ? SFN_TUNE_LDE : (int)field, log_n, batch, (int)layout}] = bs;
+  c.timing_pending = false;  // the event pair now brackets a timing run, not a caller's launch
   if (best_log_tile) *best_log_tile = bs.max_log_t;
   c.last_error.clear();
   return SFN_OK;

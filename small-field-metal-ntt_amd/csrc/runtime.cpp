@@ -987,7 +987,7 @@ static sfn_status host_call(Context &c, const uint32_t *in, size_t in_elems, uin
   float ms = 0.f;
   if ((e = hipEventElapsedTime(&ms, c.ev0, c.ev1)) != hipSuccess) return c.hip_fail(e, "event timing");
   if (device_ns) *device_ns = (uint64_t)((double)ms * 1e6);
-  c.timing_pending = false;
+  c.timing_pending = true;
   return SFN_OK;
 }
 
@@ -1211,13 +1211,11 @@ sfn_status sfn_ntt_device(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sf
   if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
   (void)hipSetDevice(c.device);
   hipStream_t s = pick_stream(c, stream);
-  const bool own = s == c.stream;
-  if (own) (void)hipEventRecord(c.ev0, s);
+  const bool own = s == c.stream, timed = own && c.call_timing;
+  if (timed) (void)hipEventRecord(c.ev0, s);
   st = c.run_ntt(field, variant, dir, log_n, batch, layout, d_in, d_out, s);
-  if (own) {
-    (void)hipEventRecord(c.ev1, s);
-    c.timing_pending = st == SFN_OK;
-  }
+  if (timed) (void)hipEventRecord(c.ev1, s);
+  if (own) c.timing_pending = timed && st == SFN_OK;
   return st;
 }
 
@@ -1232,13 +1230,11 @@ sfn_status sfn_coset_lde_batch_device(sfn_ctx *ctx, uint32_t log_n, uint32_t bat
   if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
   (void)hipSetDevice(c.device);
   hipStream_t s = pick_stream(c, stream);
-  const bool own = s == c.stream;
-  if (own) (void)hipEventRecord(c.ev0, s);
+  const bool own = s == c.stream, timed = own && c.call_timing;
+  if (timed) (void)hipEventRecord(c.ev0, s);
   st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s);
-  if (own) {
-    (void)hipEventRecord(c.ev1, s);
-    c.timing_pending = st == SFN_OK;
-  }
+  if (timed) (void)hipEventRecord(c.ev1, s);
+  if (own) c.timing_pending = timed && st == SFN_OK;
   return st;
 }
 
@@ -1253,19 +1249,26 @@ sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t b
   if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
   (void)hipSetDevice(c.device);
   hipStream_t s = pick_stream(c, stream);
-  const bool own = s == c.stream;
-  if (own) (void)hipEventRecord(c.ev0, s);
+  const bool own = s == c.stream, timed = own && c.call_timing;
+  if (timed) (void)hipEventRecord(c.ev0, s);
   st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s, nullptr, true);
-  if (own) {
-    (void)hipEventRecord(c.ev1, s);
-    c.timing_pending = st == SFN_OK;
-  }
+  if (timed) (void)hipEventRecord(c.ev1, s);
+  if (own) c.timing_pending = timed && st == SFN_OK;
   return st;
+}
+
+sfn_status sfn_call_timing(sfn_ctx *ctx, int enable) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  ctx->impl.call_timing = enable != 0;
+  return SFN_OK;
 }
 
 sfn_status sfn_last_device_ns(sfn_ctx *ctx, uint64_t *ns) {
   if (!ctx || !ns) return SFN_ERR_INVALID_ARGUMENT;
   Context &c = ctx->impl;
+  if (!c.timing_pending)
+    return c.fail(SFN_ERR_INVALID_ARGUMENT,
+                  "no timed launch sequence: the last call on the context stream failed or ran with call timing off");
   hipError_t e = hipEventSynchronize(c.ev1);
   if (e != hipSuccess) return c.hip_fail(e, "event");
   float ms = 0.f;

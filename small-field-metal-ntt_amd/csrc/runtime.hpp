@@ -57,7 +57,11 @@ class Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // (ev0, ev1) bracket the last launch sequence on `stream` (read by sfn_last_device_ns)
   bool timing_pending = false;
+  // the *_device entry points record that pair only when enabled: the two
+  // event records cost ~8-10 us of device time per call
+  bool call_timing = false;
   std::string last_error;
   hipDeviceProp_t props{};
   std::recursive_mutex mu;  // serialises table-cache updates (server threads share it)

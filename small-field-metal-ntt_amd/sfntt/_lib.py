@@ -88,6 +88,7 @@ SIGNATURES = {
                                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                                   vp, vp, vp]),
     "sfn_last_device_ns": (ctypes.c_int, [vp, u64p]),
+    "sfn_call_timing": (ctypes.c_int, [vp, ctypes.c_int]),
     "sfn_pass_timing": (ctypes.c_int, [vp, ctypes.c_int]),
     "sfn_pass_times": (ctypes.c_int, [vp, u64p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     "sfn_pass_label": (ctypes.c_char_p, [vp, ctypes.c_uint32]),

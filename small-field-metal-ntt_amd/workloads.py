@@ -49,6 +49,7 @@ def _single(ctx, field: int, log_n: int, steps: int, warmup: int) -> dict:
     ctx.check(lib.sfn_autotune(ctx.ptr, field, log_n, 1, L.LAYOUT_SOA, ctypes.byref(best)))
     res = {}
     ns = ctypes.c_uint64()
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 1))  # device time per call (event pair)
     for name, v in VARIANTS:
         dev = []
         for k in range(warmup + steps):
@@ -66,6 +67,7 @@ def _single(ctx, field: int, log_n: int, steps: int, warmup: int) -> dict:
         d, w = float(np.median(dev)), float(np.median(wall))
         res[name] = {"device_us": round(d, 2), "call_us": round(w, 1),
                      "device_melem_s": round(n / d, 1), "call_melem_s": round(n / w, 1)}
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 0))
     return res
 
 
@@ -96,6 +98,7 @@ def _batch20(ctx, steps, warmup):
     buf = sfntt.DeviceBuffer(ctx, total)
     ns = ctypes.c_uint64()
     modes = {}
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 1))  # device time per call (event pair)
     for field in (L.FIELD_KOALABEAR, L.FIELD_BABYBEAR, L.FIELD_M31):
         buf.upload(synth.field_data(field, 42 + log_n, total))
         for lay, lname in ((L.LAYOUT_SOA, "soa"), (L.LAYOUT_ROW_MAJOR, "rowmajor")):
@@ -110,6 +113,7 @@ def _batch20(ctx, steps, warmup):
             modes[f"{FIELD_TAGS[field]}_{lname}"] = {"device_us": round(d, 1),
                                                     "gelem_s": round(total / d / 1e3, 2),
                                                     "hbm_gb_s_min": round(2 * 4 * total / d / 1e3, 1)}
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 0))
     kb = modes["kb_rowmajor"]
     return {"metric": "KoalaBear forward NTT, 256 columns of 2^20 (row-major), device Gelem/s",
             "value": kb["gelem_s"], "unit": "Gelem/s", "higher_is_better": True, "dtype": "u32",
@@ -145,11 +149,12 @@ def _small(ctx, steps, warmup):
         descs = [L.JobDesc(field, L.FORWARD, log_n, 1, ctypes.c_void_p(buf.ptr.value + 4 * n * c),
                            ctypes.c_void_p(buf.ptr.value + 4 * n * c)) for c in range(cols)]
         arr = (L.JobDesc * cols)(*descs)
-        plan = ctypes.c_void_p()
-        ctx.check(lib.sfn_plan_create(ctx.ptr, arr, cols, 1, ctypes.byref(plan)))
-        dt = _timed(ctx, lambda: ctx.check(lib.sfn_plan_launch(plan, None)), steps, warmup)
-        lib.sfn_plan_destroy(plan)
-        res.update({"plan_us": round(dt * 1e6, 1), "plan_melem_s": round(total / dt / 1e6, 1)})
+        for use_graph, tag in ((1, "plan_graph"), (0, "plan_direct")):
+            plan = ctypes.c_void_p()
+            ctx.check(lib.sfn_plan_create(ctx.ptr, arr, cols, use_graph, ctypes.byref(plan)))
+            dt = _timed(ctx, lambda: ctx.check(lib.sfn_plan_launch(plan, None)), steps, warmup)
+            lib.sfn_plan_destroy(plan)
+            res.update({f"{tag}_us": round(dt * 1e6, 1), f"{tag}_melem_s": round(total / dt / 1e6, 1)})
         few = 256  # single-column launches, timed on a subset and scaled per column
         ptrs = [ctypes.c_void_p(buf.ptr.value + 4 * n * c) for c in range(few)]
         dt = _timed(ctx, lambda: [ctx.check(lib.sfn_ntt_device(ctx.ptr, field, L.VARIANT_AUTO, L.FORWARD, log_n, 1,
@@ -159,7 +164,7 @@ def _small(ctx, steps, warmup):
                     "per_column_melem_s": round(few * n / dt / 1e6, 1)})
         out[key] = res
         buf.free()
-    best = max(v["plan_melem_s"] for v in out.values())
+    best = max(max(v["plan_graph_melem_s"], v["plan_direct_melem_s"]) for v in out.values())
     return {"metric": "batched small NTTs (2^8, 2^10 points), Melem/s", "value": best, "unit": "Melem/s",
             "higher_is_better": True, "dtype": "u32", "data": "synthetic (reference LCG values)",
             "config": {"workload": "small_ntt_batches"}, "modes": out}
@@ -210,7 +215,8 @@ def _uneven(ctx, steps: int, warmup: int) -> dict:
     dt = (time.perf_counter() - t0) / steps
     out["per_column"] = {"ms": round(dt * 1e3, 3), "melem_s": round(total / dt / 1e6, 1), "launches": None}
     return {"metric": "uneven batch: 512 BabyBear+M31 columns, log_n ~ U{10..22}, forward, Melem/s",
-            "value": out["plan_graph"]["melem_s"], "unit": "Melem/s", "higher_is_better": True, "dtype": "u32",
+            "value": max(out["plan_graph"]["melem_s"], out["plan_direct"]["melem_s"]), "unit": "Melem/s",
+            "higher_is_better": True, "dtype": "u32",
             "data": "synthetic (SplitMix64 seed 0x5EED0004 shapes, reference LCG values)",
             "config": {"workload": "uneven_512_columns_log10to22", "columns": len(cols), "elements": total},
             "modes": out}

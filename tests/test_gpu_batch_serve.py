@@ -175,6 +175,40 @@ def test_autotune_and_tuning_roundtrip(ctx):
         ctx.check(ctx._lib.sfn_tuning_load(ctx.ptr, b'[{"field":1}]'))
 
 
+def test_call_timing_is_opt_in(ctx):
+    # device-pointer calls record the per-call event pair only with sfn_call_timing on;
+    # host-pointer calls always time themselves
+    lib = ctx._lib
+    log_n = 12
+    x = orc.lcg_fill(77, 1 << log_n, orc.FIELD_BB)
+    exp = orc.ntt_batch(orc.FIELD_BB, False, log_n, 1, x)
+    din, dout = sfntt.DeviceBuffer(ctx, 1 << log_n), sfntt.DeviceBuffer(ctx, 1 << log_n)
+    din.upload(x)
+    ns = ctypes.c_uint64()
+
+    def device_call():
+        ctx.check(lib.sfn_ntt_device(ctx.ptr, L.FIELD_BABYBEAR, L.VARIANT_AUTO, L.FORWARD, log_n, 1,
+                                     L.LAYOUT_SOA, din.ptr, dout.ptr, None))
+
+    device_call()
+    with pytest.raises(sfntt.api.InvalidArgument):
+        ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+    assert np.array_equal(dout.download(), exp)
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 1))
+    try:
+        device_call()
+        ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+        assert 0 < ns.value < 10**9
+    finally:
+        ctx.check(lib.sfn_call_timing(ctx.ptr, 0))
+    assert np.array_equal(dout.download(), exp)
+    y, dev_ns = ctx.ntt(L.FIELD_BABYBEAR, L.VARIANT_AUTO, L.FORWARD, log_n, 1, x)
+    ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+    assert np.array_equal(y, exp) and ns.value == dev_ns > 0
+    din.free()
+    dout.free()
+
+
 # ------------------------------------------------------------------ serving
 def make_server(ctx, **kw):
     cfg = L.ServerConfig(kw.get("mem", 0), kw.get("inflight", 4), kw.get("queue", 0),
