@@ -202,7 +202,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not offered "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not offered "
               "(the LDE would shard by column, with no exchange between GPUs)", file=sys.stderr)
         return 2
 

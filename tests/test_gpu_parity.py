@@ -23,7 +23,7 @@ VNAMES = ["v1_ct_dit_r2", "v2_ct_gs_r2", "v3_stockham_r2", "v4_ct_gs_r4", "auto"
 
 @pytest.fixture(scope="module")
 def ctx():
-    # On the GPU box a missing device is a failure, not a skip.
+    # On a machine with the GPU a missing device is a failure, not a skip.
     return sfntt.HipContext(0)
 
 
