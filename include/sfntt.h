@@ -19,7 +19,10 @@
  *    before any GPU work; n == 1 (log_n == 0) is the identity.
  *  - A context is externally synchronised (one host thread at a time), like
  *    the reference's single-threaded backends.  The server object is
- *    internally synchronised.
+ *    internally synchronised; its workers share the context's device tables,
+ *    tuning entries and error message with the owning thread, and the library
+ *    guards those itself (a table that grows keeps its old buffer until the
+ *    context is destroyed, so a launch another thread is issuing stays valid).
  */
 #ifndef SFNTT_H
 #define SFNTT_H
@@ -83,7 +86,9 @@ typedef struct {
 /* ---- context: replaces MetalContext::new / device_info (src/gpu.rs:38-64, :137-150) ---- */
 sfn_status sfn_ctx_create(int device, sfn_ctx **out);
 void sfn_ctx_destroy(sfn_ctx *ctx);
-const char *sfn_last_error(const sfn_ctx *ctx); /* owned by ctx; "" when none */
+/* Last error message of ctx ("" when none): a copy owned by the library for the
+ * calling thread, valid until that thread calls sfn_last_error again. */
+const char *sfn_last_error(const sfn_ctx *ctx);
 const char *sfn_status_string(sfn_status s);
 sfn_status sfn_device_info_get(sfn_ctx *ctx, sfn_device_info *info);
 

@@ -188,10 +188,13 @@ sfn_status sfn_autotune(sfn_ctx *cx, sfn_field field, uint32_t log_n, uint32_t b
       bs = s;
     }
   }
-  c.tuned[sfn::TuneKey{lde ? SFN_TUNE_LDE : (int)field, log_n, batch, (int)layout}] = bs;
+  {
+    std::lock_guard<std::mutex> lk(c.tune_mu);
+    c.tuned[sfn::TuneKey{lde ? SFN_TUNE_LDE : (int)field, log_n, batch, (int)layout}] = bs;
+  }
   c.timing_pending = false;  // the event pair now brackets a timing run, not a caller's launch
   if (best_log_tile) *best_log_tile = bs.max_log_t;
-  c.last_error.clear();
+  c.clear_error();
   return SFN_OK;
 }
 
@@ -200,6 +203,7 @@ sfn_status sfn_tuning_save(sfn_ctx *cx, char *buf, size_t cap, size_t *len) {
   std::ostringstream o;
   o << "[";
   bool first = true;
+  std::lock_guard<std::mutex> lk(cx->impl.tune_mu);
   for (auto &kv : cx->impl.tuned) {
     if (!first) o << ",";
     first = false;
@@ -247,6 +251,7 @@ sfn_status sfn_tuning_load(sfn_ctx *cx, const char *json) {
     add[sfn::TuneKey{(int)f, (uint32_t)ln, (uint32_t)b, (int)lay}] = s;
     p = e + 1;
   }
+  std::lock_guard<std::mutex> lk(c.tune_mu);
   for (auto &kv : add) c.tuned[kv.first] = kv.second;
   return SFN_OK;
 }

@@ -15,12 +15,22 @@ namespace sfn {
 
 // ------------------------------------------------------------------ errors
 sfn_status Context::fail(sfn_status s, const std::string &msg) {
+  std::lock_guard<std::mutex> lk(err_mu);
   last_error = msg;
   return s;
 }
 
+void Context::clear_error() {
+  std::lock_guard<std::mutex> lk(err_mu);
+  last_error.clear();
+}
+
 sfn_status Context::hip_fail(hipError_t e, const char *what) {
-  last_error = std::string(what) + ": " + hipGetErrorString(e);
+  std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+  {
+    std::lock_guard<std::mutex> lk(err_mu);
+    last_error.swap(msg);
+  }
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SFN_ERR_BUFFER_ALLOC_FAILED;
   return SFN_ERR_GPU_EXECUTION;
 }
@@ -39,6 +49,7 @@ Context::~Context() {
   };
   for (auto &f : mf_pos)
     for (auto &b : f) rel(b);
+  for (auto &b : retired) rel(b);
   rel(bb_small_pair);
   for (auto &b : mf_root) rel(b);
   for (auto &kv : mf_twist) rel(kv.second);
@@ -133,7 +144,12 @@ sfn_status Context::position_table(int field, uint32_t log_max, const uint2 **fw
   if (mf_pos_log[fi] < log_max) {
     for (int d = 0; d < 2; ++d) {
       auto t = host::position_table(host::monty_field(field), log_max, d == 1);
-      if (mf_pos[fi][d].bytes) table_bytes -= mf_pos[fi][d].bytes;
+      // another thread may hold the smaller table's address for a launch it has
+      // not issued yet: keep that buffer until the context is destroyed
+      if (mf_pos[fi][d].ptr) {
+        retired.push_back(mf_pos[fi][d]);
+        mf_pos[fi][d] = DevBuf{};
+      }
       sfn_status s = upload_table(mf_pos[fi][d], t.data(), t.size() * sizeof(host::Pair));
       if (s) return s;
     }
@@ -294,8 +310,11 @@ sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t a
 
 // ------------------------------------------------------------------ planning
 Schedule Context::schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const {
-  auto it = tuned.find(TuneKey{field, log_n, batch, layout});
-  if (it != tuned.end()) return it->second;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tuned.find(TuneKey{field, log_n, batch, layout});
+    if (it != tuned.end()) return it->second;
+  }
   Schedule s;
   // keep at least ~1024 workgroups per pass: small single columns get smaller tiles
   const uint64_t elems = (uint64_t)batch << log_n;
@@ -1048,7 +1067,14 @@ void sfn_ctx_destroy(sfn_ctx *ctx) {
   delete ctx;
 }
 
-const char *sfn_last_error(const sfn_ctx *ctx) { return ctx ? ctx->impl.last_error.c_str() : ""; }
+const char *sfn_last_error(const sfn_ctx *ctx) {
+  if (!ctx) return "";
+  // a per-thread copy: another thread may replace the context's message meanwhile
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(ctx->impl.err_mu);
+  copy = ctx->impl.last_error;
+  return copy.c_str();
+}
 
 sfn_status sfn_device_info_get(sfn_ctx *ctx, sfn_device_info *info) {
   if (!ctx || !info) return SFN_ERR_INVALID_ARGUMENT;

@@ -62,7 +62,8 @@ class Context {
   // the *_device entry points record that pair only when enabled: the two
   // event records cost ~8-10 us of device time per call
   bool call_timing = false;
-  std::string last_error;
+  std::string last_error;  // written under err_mu (server workers share the context)
+  mutable std::mutex err_mu;
   hipDeviceProp_t props{};
   std::recursive_mutex mu;  // serialises table-cache updates (server threads share it)
 
@@ -81,6 +82,7 @@ class Context {
   std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, int>, std::pair<DevBuf, DevBuf>>
       bb_lde_fast_tab;  // (L, a, shift, lo, low_major)
   size_t table_bytes = 0;
+  std::vector<DevBuf> retired;  // outgrown position tables, freed with the context
 
   // scratch for the host-pointer API and V3 ping-pong
   DevBuf scratch_in, scratch_out, scratch_pp;
@@ -91,6 +93,7 @@ class Context {
   bool scratch_ev_used = false;
 
   std::map<TuneKey, Schedule> tuned;
+  mutable std::mutex tune_mu;  // guards `tuned` (server workers read it while a caller tunes)
   bool allow_fast = true;  // SFNTT_GENERIC_ONLY=1 at context creation turns the tuned kernels off
   bool xcd_remap = true;   // SFNTT_XCD_REMAP=0 at context creation keeps workgroup id = tile
 
@@ -107,6 +110,7 @@ class Context {
   ~Context();
   sfn_status fail(sfn_status s, const std::string &msg);
   sfn_status hip_fail(hipError_t e, const char *what);
+  void clear_error();
 
   sfn_status ensure(DevBuf &b, size_t bytes);
   // scratch_lde of at least `bytes`, ordered after its previous user on any stream

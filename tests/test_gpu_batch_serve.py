@@ -297,6 +297,38 @@ def test_serving_koalabear_ntt_jobs(ctx):
     ctx._lib.sfn_server_destroy(srv)
 
 
+def test_server_and_owner_thread_share_tables():
+    # The server's workers and the owning thread use one context at once: the
+    # workers run small NTTs (log_n 3..5 take the generic kernels and their
+    # position tables) while the owner runs generic variants at growing sizes,
+    # so the position tables grow under the workers, and re-tunes a shape.
+    c = sfntt.HipContext(0)
+    srv = make_server(c, inflight=4)
+    keep = []
+    for i in range(64):
+        f = (1, 2, 0)[i % 3]
+        log_n, d = 3 + i % 9, (i // 3) % 2
+        x = synth.field_data(f, 700 + i, 1 << log_n)
+        out = np.zeros_like(x)
+        r = L.Request(L.JOB_NTT, f, d, log_n, 1, 0, 0, 0, x.ctypes.data, out.ctypes.data)
+        t = ctypes.c_uint64()
+        c.check(c._lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+        keep.append((t.value, f, log_n, d, x, out))
+    best = ctypes.c_uint32()
+    for log_n in (12, 14, 16, 18, 20):
+        for f in (L.FIELD_BABYBEAR, L.FIELD_KOALABEAR):
+            x = synth.field_data(f, 900 + log_n, 1 << log_n)
+            for v in (L.CT_GS_R2, L.CT_GS_R4):
+                y, _ = c.ntt(f, v, L.FORWARD, log_n, 1, x)
+                assert np.array_equal(y, orc.ntt_batch(f, False, log_n, 1, x)), (f, v, log_n)
+        c.check(c._lib.sfn_autotune(c.ptr, L.FIELD_BABYBEAR, 8, 1, L.LAYOUT_SOA, ctypes.byref(best)))
+    for t, f, log_n, d, x, out in keep:
+        c.check(c._lib.sfn_server_wait(srv, t, None))
+        assert np.array_equal(out, orc.ntt_batch(f, d == 1, log_n, 1, x)), (f, log_n, d)
+    c._lib.sfn_server_destroy(srv)
+    c.close()
+
+
 def test_serving_admission_and_errors(ctx):
     srv = make_server(ctx, inflight=1, mem=1 << 20, queue=2)
     big = np.zeros(1 << 20, np.uint32)
