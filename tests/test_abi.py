@@ -59,6 +59,10 @@ def test_null_context_is_rejected():
     assert lib.sfn_ntt(None, 0, 0, 0, 4, 1, 0, None, None, None) == _lib.ERR_INVALID_ARGUMENT
     assert lib.sfn_coset_lde_batch(None, 4, 1, 1, 0, 0, None, None, None) == _lib.ERR_INVALID_ARGUMENT
     assert lib.sfn_last_error(None) == b""
+    assert lib.sfn_call_timing(None, 1) == _lib.ERR_INVALID_ARGUMENT
+    ns = ctypes.c_uint64()
+    assert lib.sfn_last_device_ns(None, ctypes.byref(ns)) == _lib.ERR_INVALID_ARGUMENT
+    assert lib.sfn_pass_timing(None, 1) == _lib.ERR_INVALID_ARGUMENT
 
 
 def test_python_api_raises_device_not_found_or_works():
