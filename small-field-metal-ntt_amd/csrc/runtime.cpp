@@ -301,7 +301,6 @@ sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t a
   if (layout != SFN_LAYOUT_SOA && layout != SFN_LAYOUT_ROW_MAJOR)
     return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown layout");
   if (batch == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "batch must be >= 1");
-  if (log_n == 0) return ctx->fail(SFN_ERR_INVALID_SIZE, "coset LDE needs log_n >= 1");
   // shift = two_adic_generator(log_n + added_bits + 1) needs <= 27 (src/lde/mod.rs:59)
   if (log_n + added_bits + 1 > 27) return ctx->fail(SFN_ERR_INVALID_SIZE, "log_n + added_bits + 1 exceeds 27");
   if (((uint64_t)batch << (log_n + added_bits)) > (1ull << 34)) return ctx->fail(SFN_ERR_INVALID_SIZE, "LDE too large");
@@ -668,6 +667,18 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   const uint32_t shift_std =
       shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
   timing_begin_sequence(s);
+  if (log_n == 0) {
+    // one row per column: the interpolant is that value, so every output row
+    // repeats it (the reference's iDFT of one point is a no-op, its shift power
+    // s^0 = 1, and the forward DFT of (c, 0, .., 0) is c everywhere).  The row
+    // copy with a zero source row stride broadcasts the input row.
+    const Layout Lo = layout_of(layout, LN, batch);
+    (void)timing_mark(s, true, "k_rows_bitrev<broadcast>");
+    hipError_t e = launch_rows_bitrev(d_in, d_out, LN, batch, 1, 0, Lo.cs, Lo.rs, s);
+    if (e != hipSuccess) return hip_fail(e, "LDE broadcast launch");
+    (void)timing_mark(s, false, "");
+    return SFN_OK;
+  }
   Schedule sc = force ? *force : schedule_for(SFN_TUNE_LDE, log_n, batch, layout);
   // middle pass covers [lo, log_n); K1 (inverse) and K3 (forward) cover [0, lo),
   // each in one or more passes of at most max_log_t layers

@@ -280,7 +280,7 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
   size_t in_b, out_b;
   if (req->kind == SFN_JOB_LDE) {
     if (req->field != SFN_FIELD_BABYBEAR) return SFN_ERR_UNSUPPORTED;
-    if (req->log_n == 0 || req->batch == 0 || req->log_n + req->added_bits + 1 > 27) return SFN_ERR_INVALID_SIZE;
+    if (req->batch == 0 || req->log_n + req->added_bits + 1 > 27) return SFN_ERR_INVALID_SIZE;
     in_b = ((size_t)req->batch << req->log_n) * 4;
     out_b = ((size_t)req->batch << (req->log_n + req->added_bits)) * 4;
   } else if (req->kind == SFN_JOB_NTT) {

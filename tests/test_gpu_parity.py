@@ -293,6 +293,27 @@ def test_maximum_sizes(ctx, field, log_n):
     assert_same(z, x, f"round trip 2^{log_n} field={field}")
 
 
+@pytest.mark.parametrize("log_n,batch,added_bits", [(0, 3, 2), (0, 5, 0), (0, 1, 3), (8, 4, 0), (12, 6, 0),
+                                                    (14, 16, 0), (20, 4, 0)])
+@pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
+def test_lde_one_row_and_no_blowup(ctx, log_n, batch, added_bits, layout):
+    # log_n = 0 (the reference's iDFT of one point is a no-op: every output row
+    # repeats the input) and added_bits = 0 (a coset evaluation of the same size)
+    x = data(BB, batch << log_n, 910 + log_n)
+    shift = 0 if layout == sfntt.LAYOUT_SOA else orc.bb_to_monty(31)
+    print(f"[parity] lde edge log_n={log_n} batch={batch} a={added_bits} layout={layout}", flush=True)
+    exp = orc.coset_lde_batch(x, log_n, batch, added_bits, shift, layout)
+    y, _ = ctx.coset_lde(x, log_n, batch, added_bits, shift, layout)
+    assert_same(y, exp, f"LDE 2^{log_n} x {batch}, {1 << added_bits}x")
+    if log_n == 0:
+        assert np.array_equal(np.unique(y.reshape(-1, batch) if layout else y.reshape(batch, -1).T, axis=0),
+                              x.reshape(1, batch)), "every output row repeats the input row"
+        y, _ = ctx.coset_lde(x, log_n, batch, added_bits, shift, layout, natural=True)
+        assert_same(y, exp, "natural-order LDE of one row")
+        r = api.CosetLdeBatch(ctx=ctx).execute(x, log_n, batch, added_bits, shift, layout)
+        assert_same(r.data, exp, "CosetLdeBatch.execute of one row")
+
+
 @pytest.mark.parametrize("log_n,added_bits,layout", [(25, 1, sfntt.LAYOUT_SOA), (24, 2, sfntt.LAYOUT_ROW_MAJOR)],
                          ids=["2e25_2x_soa", "2e24_4x_rowmajor"])
 def test_lde_maximum_sizes(ctx, log_n, added_bits, layout):
