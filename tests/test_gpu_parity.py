@@ -114,7 +114,7 @@ def test_batches_match_single_columns(ctx, field, layout):
         exp_f = orc.ntt_batch(field, False, log_n, batch, x)
         exp_i = orc.ntt_batch(field, True, log_n, batch, x)
         xin = x if layout == sfntt.LAYOUT_SOA else x.reshape(batch, n).T.copy().ravel()
-        for variant in (sfntt.VARIANT_AUTO, sfntt.CT_GS_R4, sfntt.CT_DIT_R2):
+        for variant in VARIANTS:
             print(f"[parity] batch field={field} layout={layout} log_n={log_n} batch={batch} v={variant}",
                   flush=True)
             yf, _ = ctx.ntt(field, variant, sfntt.FORWARD, log_n, batch, xin, layout)
