@@ -326,7 +326,7 @@ def test_lde_maximum_sizes(ctx, log_n, added_bits, layout):
 
 
 @pytest.mark.parametrize("log_n,batch,added_bits", [(1, 3, 1), (6, 5, 2), (12, 8, 1), (15, 24, 1), (20, 2, 1),
-                                                    (21, 2, 2)])
+                                                    (21, 2, 2), (10, 4, 0), (13, 8, 0)])
 @pytest.mark.parametrize("shift", [0, 31], ids=["ref_shift", "generator"])
 @pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
 def test_coset_lde_natural_order(ctx, log_n, batch, added_bits, shift, layout):
