@@ -133,6 +133,7 @@ class Context {
   // Autotuned schedule of (field, log_n, batch, layout) if one is stored (field 2 =
   // the LDE), else the default for the size; field -1 always gives the default.
   Schedule schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const;
+  Schedule schedule_for_column(int field, uint32_t log_n, int layout) const;  // one AUTO column
 
   // Plans and runs a transform on device buffers.
   sfn_status run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
