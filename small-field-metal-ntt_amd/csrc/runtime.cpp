@@ -341,6 +341,39 @@ Schedule Context::schedule_for_column(int field, uint32_t log_n, int layout) con
   return s;
 }
 
+// AUTO schedule of one batched call (batch > 1, at most 2^22 elements) with
+// no tuning entry: the autotuner's MI355X choices for BabyBear and M31 at
+// 2^8..2^20 x 4..256 columns (back-to-back calls 1.3-3.3x faster than the
+// batch rule, which keeps >= ~1024 workgroups per pass and so runs more, smaller
+// passes than these launch-bound sizes want).  2^12 and SoA 2^13..2^14 keep the
+// batch rule, as do larger calls.
+Schedule Context::schedule_for_batch(int field, uint32_t log_n, uint32_t batch, int layout) const {
+  const uint64_t elems = (uint64_t)batch << log_n;
+  const bool rm = layout == SFN_LAYOUT_ROW_MAJOR;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tuned.find(TuneKey{field, log_n, batch, layout});
+    if (it != tuned.end()) return it->second;
+  }
+  if (elems > (1ull << 22) || log_n == 12 || (!rm && (log_n == 13 || log_n == 14)))
+    return schedule_for(field, log_n, batch, layout);
+  Schedule s;
+  if (log_n <= 11) {  // one pass
+    s.max_log_t = 11;
+    s.logr = rm && log_n <= 9 ? 3 : 5;
+    s.log_w = rm ? 2 : 3;
+  } else if (log_n <= 16) {  // two passes
+    s.max_log_t = 10;
+    const bool few = elems <= (1ull << 18);
+    s.logr = few ? 3 : 5;
+    s.log_w = few ? 2 : 4;
+  } else {
+    s.max_log_t = 11;
+    s.log_w = 4;
+  }
+  return s;
+}
+
 Schedule Context::schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const {
   {
     std::lock_guard<std::mutex> lk(tune_mu);
@@ -669,7 +702,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
   Schedule sc = force ? *force
                       : (variant != SFN_VARIANT_AUTO ? schedule_for(-1, log_n, batch, layout)
                          : batch == 1                  ? schedule_for_column(field, log_n, layout)
-                                                       : schedule_for(field, log_n, batch, layout));
+                                                       : schedule_for_batch(field, log_n, batch, layout));
   const bool fwd = dir == SFN_FORWARD;
   std::vector<PassDesc> passes = plan_ntt_passes(field, variant, dir, log_n, batch, layout, sc, true);
   (void)fwd;
