@@ -405,6 +405,11 @@ Schedule Context::schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) c
   return s;
 }
 
+bool Context::has_tuning(const TuneKey &k) const {
+  std::lock_guard<std::mutex> lk(tune_mu);
+  return tuned.count(k) != 0;
+}
+
 Schedule Context::schedule_for(int field, uint32_t log_n, uint32_t batch, int layout) const {
   {
     std::lock_guard<std::mutex> lk(tune_mu);
@@ -995,7 +1000,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
     // points, except that two 11-layer passes (three register rounds each)
     // lose to three shorter ones (measured on 2^22 columns).  A tuned entry
     // keeps its pass size; the radix is always the tuned kernel's.
-    if (ctx->tuned.find(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}) == ctx->tuned.end())
+    if (!ctx->has_tuning(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}))
       sc.max_log_t = m.log_n > 2 * (kMaxLogTile - 1) + 1 ? kMaxLogTile - 1 : kMaxLogTile;
     sc.logr = 5;
     jp[j] = plan_ntt_passes(m.field, SFN_VARIANT_AUTO, m.dir, m.log_n, m.batch, SFN_LAYOUT_SOA, sc, false);

@@ -136,6 +136,7 @@ class Context {
   Schedule schedule_for_column(int field, uint32_t log_n, int layout) const;  // one AUTO column
   Schedule schedule_for_batch(int field, uint32_t log_n, uint32_t batch, int layout) const;  // AUTO batch
   Schedule schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) const;                // coset LDE
+  bool has_tuning(const TuneKey &k) const;
 
   // Plans and runs a transform on device buffers.
   sfn_status run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
