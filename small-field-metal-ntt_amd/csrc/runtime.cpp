@@ -374,13 +374,16 @@ Schedule Context::schedule_for_batch(int field, uint32_t log_n, uint32_t batch, 
   return s;
 }
 
-// Coset-LDE schedule (2x and more) of a call with no tuning entry and at most
-// 2^23 input elements: the autotuner's MI355X choices at 2^8..2^20 x 1..256
-// columns (back-to-back calls 1.3-4.9x faster than the batch rule, whose small
-// tiles split these sizes into five or more launches): sub-transforms of up to
-// 2^11 points, so the middle pass takes half the layers and each outer side one
-// pass.  SoA 2^9..2^10 prefers radix-8 outer passes; 2^8 (but row-major
-// batches of >= 8 columns) and smaller keep the batch rule, as do larger calls.
+// Coset-LDE schedule (2x and more) of a call with no tuning entry: the
+// autotuner's MI355X choices at 2^8..2^20 x 1..4096 columns.  Calls of at most
+// 2^23 input elements use sub-transforms of up to 2^11 points, so the middle
+// pass takes half the layers and each outer side one pass (back-to-back calls
+// 1.1-4.8x faster than the batch rule, whose small tiles split these sizes
+// into five or more launches); SoA 2^9..2^10 prefers radix-8 outer passes, and
+// 2^8 (but row-major batches of >= 8 columns) and smaller keep the batch rule.
+// At every size the middle pass takes as many lanes as its LDS allows three
+// workgroups per CU: 32 for tiles of <= 2^8 points, 16 for 2^9, 8 for 2^10 and
+// up (larger calls 1.07-2.4x faster, the 2^20 x 256 LDE unchanged).
 Schedule Context::schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) const {
   const uint64_t elems = (uint64_t)batch << log_n;
   const bool rm = layout == SFN_LAYOUT_ROW_MAJOR;
@@ -389,19 +392,21 @@ Schedule Context::schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) c
     auto it = tuned.find(TuneKey{SFN_TUNE_LDE, log_n, batch, layout});
     if (it != tuned.end()) return it->second;
   }
-  if (elems > (1ull << 23) || log_n < 8 || (log_n == 8 && (!rm || batch < 8)))
-    return schedule_for(SFN_TUNE_LDE, log_n, batch, layout);
   Schedule s;
-  s.max_log_t = 11;
-  s.log_w = 4;
-  // 2^10-point middle tiles keep 8 lanes (three workgroups per CU fit in LDS)
-  const uint32_t mid_t = log_n <= 11 ? log_n : (log_n + 1) / 2;
-  s.mid_log_w = mid_t >= 10 ? 3 : 4;
-  if (!rm && log_n <= 10) {
-    s.max_log_t = 10;
-    s.logr = 3;
-    s.log_w = 2;
+  if (elems > (1ull << 23) || log_n < 8 || (log_n == 8 && (!rm || batch < 8))) {
+    s = schedule_for(SFN_TUNE_LDE, log_n, batch, layout);
+  } else {
+    s.max_log_t = 11;
+    s.log_w = 4;
+    if (!rm && log_n <= 10) {
+      s.max_log_t = 10;
+      s.logr = 3;
+      s.log_w = 2;
+    }
   }
+  const uint32_t mlt = std::min<uint32_t>(s.max_log_t, kMaxLogTile);
+  const uint32_t mid_t = std::min<uint32_t>(log_n <= mlt ? log_n : (log_n + 1) / 2, mlt);
+  s.mid_log_w = mid_t >= 10 ? 3 : mid_t == 9 ? 4 : 5;
   return s;
 }
 
