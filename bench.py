@@ -179,6 +179,39 @@ def cpu_baseline_ntt(workload: str) -> dict:
             "host": host_cpu()}
 
 
+def dispersion(samples_ms) -> dict:
+    """Median, min, max, mean, stddev, p95 and coefficient of variation of
+    per-step times (benches/ntt_benchmark.rs:75-120 reports the same set)."""
+    import numpy as np
+
+    a = np.asarray(samples_ms, dtype=np.float64)
+    mean = float(a.mean())
+    sd = float(a.std(ddof=1)) if a.size > 1 else 0.0
+    cv = 100.0 * sd / mean if mean > 0 else 0.0
+    return {"median_ms": round(float(np.median(a)), 4), "min_ms": round(float(a.min()), 4),
+            "max_ms": round(float(a.max()), 4), "mean_ms": round(mean, 4), "stddev_ms": round(sd, 4),
+            "p95_ms": round(float(np.percentile(a, 95)), 4), "cv_pct": round(cv, 2),
+            "high_variance": bool(cv > 10.0), "samples": int(a.size)}
+
+
+def step_dispersion(ctx, step, steps: int) -> dict:
+    """Device time of each step on its own (sfn_call_timing event pair)."""
+    lib = ctx._lib
+    ns = ctypes.c_uint64()
+    samples = []
+    ctx.check(lib.sfn_call_timing(ctx.ptr, 1))
+    try:
+        for _ in range(max(steps, 2)):
+            step()
+            ctx.check(lib.sfn_last_device_ns(ctx.ptr, ctypes.byref(ns)))
+            samples.append(ns.value / 1e6)
+    finally:
+        ctx.check(lib.sfn_call_timing(ctx.ptr, 0))
+    res = dispersion(samples)
+    res["source"] = "hipEvent pair around each step, device time"
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +292,11 @@ def main() -> int:
     ms_per_step = elapsed / args.steps * 1e3
     value = N * B * args.steps / elapsed / 1e6
 
+    # dispersion: each step alone between a hipEvent pair on the launch stream
+    # (the reference reports median, min, max, stddev, p95 and CV over its
+    # timed runs and flags CV above 10 %, benches/ntt_benchmark.rs:75-120)
+    disp = step_dispersion(ctx, step, args.steps)
+
     # per-kernel durations with hipEvent pairs on the launch stream
     lib.sfn_pass_timing(ctx.ptr, 1)
     per = {}
@@ -326,6 +364,9 @@ def main() -> int:
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            # HBM bytes per launch of this kernel: rocprofv3 --pmc counters of the
+            # same workload, read from the committed summary (not measured in this run)
+            "traffic_source": os.path.relpath(COUNTERS_FILE, ROOT) if traffic is not None else None,
         },
         # the same kernel against the VALU issue peak (the bound it is closest to)
         "roofline_valu": None if valu_insts is None else {
@@ -341,6 +382,7 @@ def main() -> int:
             "device_ms_events": round(sum(seq_ns) / len(seq_ns) / 1e6, 4),
             "kernels_us": {k: round(v, 1) for k, v in kernels.items()},
         },
+        "dispersion": disp,
         "cpu_baseline": cb,
         "cpu_baseline_all_cores": cb_all,
         "end_to_end": e2e,

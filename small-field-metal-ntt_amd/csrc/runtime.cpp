@@ -297,6 +297,11 @@ sfn_status validate_ntt(Context *ctx, int field, int variant, int dir, uint32_t 
   return SFN_OK;
 }
 
+bool ranges_overlap(const void *a, size_t a_bytes, const void *b, size_t b_bytes) {
+  const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+  return a_bytes && b_bytes && a0 < b0 + b_bytes && b0 < a0 + a_bytes;
+}
+
 sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout) {
   if (layout != SFN_LAYOUT_SOA && layout != SFN_LAYOUT_ROW_MAJOR)
     return ctx->fail(SFN_ERR_INVALID_ARGUMENT, "unknown layout");
@@ -1372,7 +1377,10 @@ sfn_status sfn_coset_lde_batch_device(sfn_ctx *ctx, uint32_t log_n, uint32_t bat
   sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
   if (st) return st;
   if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
-  if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
+  // the first pass reads column b at b*n and writes it at b*N: an overlapping
+  // output would overwrite input columns not read yet
+  if (sfn::ranges_overlap(d_in, ((size_t)batch << log_n) * 4, d_out, ((size_t)batch << (log_n + added_bits)) * 4))
+    return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
   (void)hipSetDevice(c.device);
   hipStream_t s = pick_stream(c, stream);
   const bool own = s == c.stream, timed = own && c.call_timing;
@@ -1391,7 +1399,10 @@ sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t b
   sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
   if (st) return st;
   if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
-  if (d_in == d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
+  // the first pass reads column b at b*n and writes it at b*N: an overlapping
+  // output would overwrite input columns not read yet
+  if (sfn::ranges_overlap(d_in, ((size_t)batch << log_n) * 4, d_out, ((size_t)batch << (log_n + added_bits)) * 4))
+    return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
   (void)hipSetDevice(c.device);
   hipStream_t s = pick_stream(c, stream);
   const bool own = s == c.stream, timed = own && c.call_timing;

@@ -180,6 +180,8 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
 uint32_t multi_grid(const Context *ctx, const MultiLaunch &L);
 hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s);
 sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout);
+// [a, a + a_bytes) and [b, b + b_bytes) share a byte
+bool ranges_overlap(const void *a, size_t a_bytes, const void *b, size_t b_bytes);
 
 }  // namespace sfn
 

@@ -296,6 +296,8 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
   (void)st;
   (void)c;
   if (!req->in || !req->out) return SFN_ERR_INVALID_ARGUMENT;
+  // an LDE writes column b at b*N while later columns are still to be read
+  if (req->kind == SFN_JOB_LDE && sfn::ranges_overlap(req->in, in_b, req->out, out_b)) return SFN_ERR_INVALID_ARGUMENT;
   auto job = std::make_unique<Job>();
   job->req = *req;
   job->in_bytes = in_b;

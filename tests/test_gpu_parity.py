@@ -128,7 +128,11 @@ def test_batches_match_single_columns(ctx, field, layout):
 
 @pytest.mark.parametrize("log_n,batch,added_bits", [(8, 4, 1), (12, 8, 1), (8, 4, 2), (1, 3, 1),
                                                     (5, 2, 3), (14, 6, 2), (17, 3, 1),
-                                                    (20, 1, 1), (21, 2, 2), (15, 40, 1)])
+                                                    (20, 1, 1), (21, 2, 2), (15, 40, 1),
+                                                    # middle-pass widths of the untuned schedule on
+                                                    # row-major columns: 32 lanes (tiles <= 2^8),
+                                                    # 16 lanes (2^9), 8 lanes (2^10)
+                                                    (12, 32, 1), (14, 64, 1), (17, 16, 1), (18, 24, 1)])
 @pytest.mark.parametrize("shift", [0, 31], ids=["ref_shift", "generator"])
 @pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
 def test_coset_lde_small(ctx, log_n, batch, added_bits, shift, layout):
