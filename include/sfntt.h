@@ -146,7 +146,16 @@ sfn_status sfn_synchronize(sfn_ctx *ctx, void *stream);
  * which keeps only the last log_n on the host.  Tables stay device-resident,
  * keyed by (field, log_n); this call builds them ahead of time. */
 sfn_status sfn_twiddles_prepare(sfn_ctx *ctx, sfn_field field, uint32_t log_n);
+/* Bytes of device tables the context holds now. */
 sfn_status sfn_twiddles_bytes(sfn_ctx *ctx, size_t *bytes);
+/* Byte budget of the per-shape tables (four-step twists, LDE factor tables
+ * keyed by the shift, M31 block twiddles); 0 restores the default, 4 GiB.
+ * Past it the least recently used tables are freed (after waiting for the
+ * device), except those an unfinished call or a live plan holds; a table
+ * larger than the budget is still admitted alone.  The reference's cache is
+ * bounded the same way in spirit: it keeps one log_n
+ * (src/ntt/twiddle_cache.rs:29-82). */
+sfn_status sfn_twiddles_limit(sfn_ctx *ctx, size_t max_bytes);
 
 sfn_status sfn_ntt_device(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sfn_direction dir,
                           uint32_t log_n, uint32_t batch, sfn_layout layout, const uint32_t *d_in,

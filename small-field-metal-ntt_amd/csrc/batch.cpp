@@ -28,8 +28,10 @@ struct sfn_plan {
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
   uint64_t total_tiles = 0;
+  std::vector<sfn::TableKey> pinned;  // cached tables the descriptors point to
 
   ~sfn_plan() {
+    if (ctx && !pinned.empty()) ctx->unpin(pinned);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -68,8 +70,10 @@ sfn_status sfn_plan_create(sfn_ctx *cx, const sfn_job_desc *jobs, uint32_t count
     mj.push_back(sfn::MultiJob{jd.field, jd.dir, jd.log_n, jd.batch, jd.d_in, jd.d_out});
   }
   std::vector<sfn::MultiLaunch> ml;
+  sfn::TablePins pins(&c);  // the plan keeps every table its descriptors name
   sfn_status bst = sfn::build_multi(&c, mj, ml);
   if (bst) return bst;
+  plan->pinned = pins.take();
   for (auto &m : ml) {
     sfn_plan::Launch L;
     L.m = std::move(m);

@@ -313,6 +313,22 @@ __global__ __launch_bounds__(256) void k_pointwise(const uint32_t *a, const uint
     out[i] = F::mul(a[i], b[i]);
 }
 
+// Segment k copies words [begin_k, begin_{k+1}) of one contiguous range (the
+// last segment ends at `total`) between its own source and destination.
+__global__ __launch_bounds__(256) void k_copy_segments(const CopySeg *segs, uint32_t nseg, uint64_t total) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nseg;  // segs[lo].begin <= i < segs[hi].begin
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (segs[mid].begin <= i) lo = mid;
+      else hi = mid;
+    }
+    const uint64_t k = i - segs[lo].begin;
+    segs[lo].dst[k] = segs[lo].src[k];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_copy(const uint32_t *src, uint32_t *dst, uint64_t len) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -526,6 +542,14 @@ hipError_t launch_rows_bitrev(const uint32_t *src, uint32_t *dst, uint32_t L, ui
   else
     hipLaunchKernelGGL(k_rows_bitrev, dim3((uint32_t)blocks), dim3(256), 0, s, src, dst, L, batch, scs, srs, dcs, drs,
                        total);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_segments(const CopySeg *d_segs, uint32_t nseg, uint64_t total, hipStream_t s) {
+  if (total == 0 || nseg == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy_segments, dim3((uint32_t)blocks), dim3(256), 0, s, d_segs, nseg, total);
   return hipGetLastError();
 }
 

@@ -109,6 +109,15 @@ constexpr int kMultiFastThreads = 256;
 hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
                              uint32_t xs, const void *bb_root, uint32_t lanes_bytes, hipStream_t s);
 hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);
+// One launch copying many ranges: word i of [segs[k].begin, segs[k+1].begin)
+// (the last ends at total) goes from segs[k].src to segs[k].dst, both indexed
+// from the segment's start.  segs is a device array of nseg entries, begin ascending.
+struct CopySeg {
+  const uint32_t *src;
+  uint32_t *dst;
+  uint64_t begin;
+};
+hipError_t launch_copy_segments(const CopySeg *d_segs, uint32_t nseg, uint64_t total, hipStream_t s);
 // dst row r = src row bitrev_L(r) for a 2^L-row, `batch`-column array with the
 // given (column, row) strides; src == dst permutes in place (same strides).
 hipError_t launch_rows_bitrev(const uint32_t *src, uint32_t *dst, uint32_t L, uint32_t batch, uint64_t scs,

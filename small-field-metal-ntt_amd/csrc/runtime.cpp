@@ -52,13 +52,9 @@ Context::~Context() {
   for (auto &b : retired) rel(b);
   rel(bb_small_pair);
   for (auto &b : mf_root) rel(b);
-  for (auto &kv : mf_twist) rel(kv.second);
-  for (auto &kv : bb_lde_mid) rel(kv.second);
-  for (auto &kv : m31_tw) rel(kv.second);
-  for (auto &kv : mf_twist_m) rel(kv.second);
-  for (auto &kv : bb_lde_fast_tab) {
-    rel(kv.second.first);
-    rel(kv.second.second);
+  for (auto &kv : tables) {
+    rel(kv.second.buf[0]);
+    rel(kv.second.buf[1]);
   }
   rel(scratch_in);
   rel(scratch_out);
@@ -112,10 +108,120 @@ sfn_status Context::ensure(DevBuf &b, size_t bytes) {
 sfn_status Context::upload_table(DevBuf &b, const void *host, size_t bytes) {
   sfn_status s = ensure(b, bytes);
   if (s) return s;
-  SFN_HIP(hipMemcpy(b.ptr, host, bytes, hipMemcpyHostToDevice), "table upload");
+  hipError_t e = hipMemcpy(b.ptr, host, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(b.ptr);  // nothing has read it yet
+    b = DevBuf{};
+    return hip_fail(e, "table upload");
+  }
   table_bytes += bytes;
   return SFN_OK;
 }
+
+void Context::release_table(DevBuf &b) {
+  if (!b.ptr) return;
+  (void)hipFree(b.ptr);
+  table_bytes -= b.bytes;
+  b = DevBuf{};
+}
+
+// ------------------------------------------------------------------ table cache
+namespace {
+thread_local TablePins *t_pins = nullptr;
+}
+
+TablePins::TablePins(Context *ctx) : ctx_(ctx), prev_(t_pins) { t_pins = this; }
+
+TablePins::~TablePins() {
+  t_pins = prev_;
+  if (!keys_.empty()) ctx_->unpin(keys_);
+}
+
+std::vector<TableKey> TablePins::take() {
+  std::vector<TableKey> k;
+  k.swap(keys_);
+  return k;
+}
+
+void Context::unpin(const std::vector<TableKey> &keys) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  for (const TableKey &k : keys) {
+    auto it = tables.find(k);
+    if (it != tables.end() && it->second.pins) --it->second.pins;
+  }
+}
+
+void Context::evict_for(size_t incoming) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  std::vector<std::map<TableKey, TableEntry>::iterator> victims;
+  size_t keep = cached_bytes;
+  while (keep + incoming > table_limit) {
+    auto lru = tables.end();
+    for (auto it = tables.begin(); it != tables.end(); ++it) {
+      if (it->second.pins || std::find(victims.begin(), victims.end(), it) != victims.end()) continue;
+      if (lru == tables.end() || it->second.last_use < lru->second.last_use) lru = it;
+    }
+    if (lru == tables.end()) break;  // everything left is pinned
+    victims.push_back(lru);
+    keep -= lru->second.bytes;
+  }
+  if (victims.empty()) return;
+  // kernels already issued on any stream may still read these tables
+  (void)hipDeviceSynchronize();
+  for (auto it : victims) {
+    release_table(it->second.buf[0]);
+    release_table(it->second.buf[1]);
+    cached_bytes -= it->second.bytes;
+    tables.erase(it);
+    ++evictions;
+  }
+}
+
+sfn_status Context::cached_table(const TableKey &k, TableBuilder build, const void **p0, const void **p1) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  auto it = tables.find(k);
+  if (it == tables.end()) {
+    std::vector<uint32_t> h0, h1;
+    build(k, h0, h1);
+    const size_t need = (h0.size() + h1.size()) * 4;
+    evict_for(need);
+    TableEntry e;
+    sfn_status s = upload_table(e.buf[0], h0.data(), h0.size() * 4);
+    if (!s && !h1.empty()) s = upload_table(e.buf[1], h1.data(), h1.size() * 4);
+    if (s) {
+      release_table(e.buf[0]);
+      release_table(e.buf[1]);
+      return s;
+    }
+    e.bytes = need;
+    cached_bytes += need;
+    it = tables.emplace(k, e).first;
+  }
+  it->second.last_use = ++table_clock;
+  if (t_pins && t_pins->ctx() == this) {
+    ++it->second.pins;
+    t_pins->add(k);
+  }
+  *p0 = it->second.buf[0].ptr;
+  if (p1) *p1 = it->second.buf[1].ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::set_table_limit(size_t bytes) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  table_limit = bytes ? bytes : kDefaultTableLimit;
+  evict_for(0);
+  return SFN_OK;
+}
+
+namespace {
+// Shoup pairs as consecutive (w, q) words
+void pair_words(const std::vector<host::Pair> &v, std::vector<uint32_t> &out) {
+  static_assert(sizeof(host::Pair) == 8, "two words per pair");
+  out.resize(v.size() * 2);
+  if (!v.empty()) memcpy(out.data(), v.data(), v.size() * 8);
+}
+}  // namespace
 
 hipError_t Context::timing_mark(hipStream_t s, bool before, const std::string &label) {
   if (!pass_timing || s != stream) return hipSuccess;
@@ -188,93 +294,68 @@ sfn_status Context::root_table(int field, const uint2 **out) {
 
 sfn_status Context::twist_m_table(int field, uint32_t lo, uint32_t hi, bool inverse, bool low_major,
                                   const uint32_t **out) {
-  std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(mf_index(field), lo, hi, (int)inverse, (int)low_major);
-  auto it = mf_twist_m.find(key);
-  if (it == mf_twist_m.end()) {
-    const host::Monty &f = host::monty_field(field);
-    auto t = host::to_monty_words(f, host::twist_table(f, lo, hi, inverse));
-    if (low_major) t = host::low_major(t, lo, hi);
-    DevBuf b;
-    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
-    if (s) return s;
-    it = mf_twist_m.emplace(key, b).first;
-  }
-  *out = (const uint32_t *)it->second.ptr;
-  return SFN_OK;
+  const TableKey k{kTwistMont, (uint32_t)field, lo, hi, (uint32_t)inverse, (uint32_t)low_major};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &) {
+        const host::Monty &f = host::monty_field((int)k.a);
+        auto t = host::to_monty_words(f, host::twist_table(f, k.b, k.c, k.d != 0));
+        if (k.e) t = host::low_major(t, k.b, k.c);
+        h0 = std::move(t);
+      },
+      (const void **)out);
 }
 
 sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, bool low_major,
                                 const uint32_t **post, const uint32_t **pointf) {
-  std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(log_n, added_bits, shift_std, lo, (int)low_major);
-  auto it = bb_lde_fast_tab.find(key);
-  if (it == bb_lde_fast_tab.end()) {
-    std::vector<uint32_t> po, pf;
-    host::bb_lde_fast_tables(log_n, added_bits, shift_std, lo, po, pf);
-    if (low_major) {  // every block's post table in (low, p) order
-      const size_t n = (size_t)1 << log_n;
-      for (size_t b = 0; b < po.size() / n; ++b) {
-        std::vector<uint32_t> blk(po.begin() + b * n, po.begin() + (b + 1) * n);
-        blk = host::low_major(blk, lo, log_n);
-        std::copy(blk.begin(), blk.end(), po.begin() + b * n);
-      }
-    }
-    DevBuf a, b;
-    sfn_status s = upload_table(a, po.data(), po.size() * 4);
-    if (s) return s;
-    s = upload_table(b, pf.data(), pf.size() * 4);
-    if (s) return s;
-    it = bb_lde_fast_tab.emplace(key, std::make_pair(a, b)).first;
-  }
-  *post = (const uint32_t *)it->second.first.ptr;
-  *pointf = (const uint32_t *)it->second.second.ptr;
-  return SFN_OK;
+  const TableKey k{kLdeFast, log_n, added_bits, shift_std, lo, (uint32_t)low_major};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &h1) {
+        std::vector<uint32_t> po, pf;
+        host::bb_lde_fast_tables(k.a, k.b, k.c, k.d, po, pf);
+        if (k.e) {  // every block's post table in (low, p) order
+          const size_t n = (size_t)1 << k.a;
+          for (size_t b = 0; b < po.size() / n; ++b) {
+            std::vector<uint32_t> blk(po.begin() + b * n, po.begin() + (b + 1) * n);
+            blk = host::low_major(blk, k.d, k.a);
+            std::copy(blk.begin(), blk.end(), po.begin() + b * n);
+          }
+        }
+        h0 = std::move(po);
+        h1 = std::move(pf);
+      },
+      (const void **)post, (const void **)pointf);
 }
 
 sfn_status Context::twist_table(int field, uint32_t lo, uint32_t hi, bool inverse, const uint2 **out) {
-  std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(mf_index(field), lo, hi, (int)inverse);
-  auto it = mf_twist.find(key);
-  if (it == mf_twist.end()) {
-    auto t = host::twist_table(host::monty_field(field), lo, hi, inverse);
-    DevBuf b;
-    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(host::Pair));
-    if (s) return s;
-    it = mf_twist.emplace(key, b).first;
-  }
-  *out = (const uint2 *)it->second.ptr;
-  return SFN_OK;
+  const TableKey k{kTwistPair, (uint32_t)field, lo, hi, (uint32_t)inverse, 0};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &) {
+        pair_words(host::twist_table(host::monty_field((int)k.a), k.b, k.c, k.d != 0), h0);
+      },
+      (const void **)out);
 }
 
 sfn_status Context::bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out) {
-  std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_tuple(log_n, added_bits, shift_std);
-  auto it = bb_lde_mid.find(key);
-  if (it == bb_lde_mid.end()) {
-    auto t = host::bb_lde_mid_table(log_n, added_bits, shift_std);
-    DevBuf b;
-    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(host::Pair));
-    if (s) return s;
-    it = bb_lde_mid.emplace(key, b).first;
-  }
-  *out = (const uint2 *)it->second.ptr;
-  return SFN_OK;
+  const TableKey k{kLdeMid, log_n, added_bits, shift_std, 0, 0};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &) {
+        pair_words(host::bb_lde_mid_table(k.a, k.b, k.c), h0);
+      },
+      (const void **)out);
 }
 
 sfn_status Context::m31_table(uint32_t log_n, bool inverse, const uint32_t **out) {
-  std::lock_guard<std::recursive_mutex> g(mu);
-  auto key = std::make_pair(log_n, (int)inverse);
-  auto it = m31_tw.find(key);
-  if (it == m31_tw.end()) {
-    auto t = host::m31_block_table(log_n, inverse);
-    DevBuf b;
-    sfn_status s = upload_table(b, t.data(), t.size() * sizeof(uint32_t));
-    if (s) return s;
-    it = m31_tw.emplace(key, b).first;
-  }
-  *out = (const uint32_t *)it->second.ptr;
-  return SFN_OK;
+  const TableKey k{kM31Block, log_n, (uint32_t)inverse, 0, 0, 0};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &) {
+        h0 = host::m31_block_table(k.a, k.b != 0);
+      },
+      (const void **)out);
 }
 
 // ------------------------------------------------------------------ validation
@@ -735,6 +816,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
 
 sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uint32_t batch, int layout,
                             const uint32_t *d_in, uint32_t *d_out, hipStream_t s, const Schedule *force) {
+  TablePins pins(this);  // the tables this call's kernels read stay resident while they are issued
   const uint64_t total = (uint64_t)batch << log_n;
   if (log_n == 0) {  // identity (src/ntt/metal_r2.rs:81-83)
     if (d_in != d_out) {
@@ -776,6 +858,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
 sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits, uint32_t shift_raw,
                             int layout, const uint32_t *d_in, uint32_t *d_out, hipStream_t s,
                             const Schedule *force, bool natural) {
+  TablePins pins(this);
   const uint32_t LN = log_n + added_bits;
   const uint32_t shift_std =
       shift_raw ? host::bb_from_monty(shift_raw) : host::bb_root(LN + 1);  // src/lde/mod.rs:59
@@ -1348,8 +1431,15 @@ sfn_status sfn_twiddles_prepare(sfn_ctx *ctx, sfn_field field, uint32_t log_n) {
 
 sfn_status sfn_twiddles_bytes(sfn_ctx *ctx, size_t *bytes) {
   if (!ctx || !bytes) return SFN_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::recursive_mutex> g(ctx->impl.mu);
   *bytes = ctx->impl.table_bytes;
   return SFN_OK;
+}
+
+sfn_status sfn_twiddles_limit(sfn_ctx *ctx, size_t max_bytes) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->impl.device);
+  return ctx->impl.set_table_limit(max_bytes);
 }
 
 sfn_status sfn_ntt_device(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sfn_direction dir, uint32_t log_n,

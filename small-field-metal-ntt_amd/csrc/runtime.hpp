@@ -43,6 +43,47 @@ struct Schedule {
   uint32_t mid_log_w = 3;   // lanes per workgroup of the LDE middle pass
 };
 
+// Key of a per-shape device table in the context's bounded cache.
+enum TableKind : uint32_t {
+  kTwistPair = 1,   // four-step twist, Shoup pairs: (field, lo, hi, inverse)
+  kTwistMont = 2,   // four-step twist, Montgomery words: (field, lo, hi, inverse, low_major)
+  kLdeMid = 3,      // generic LDE middle-pass blocks: (log_n, added_bits, shift)
+  kLdeFast = 4,     // tuned LDE post / point-factor tables: (log_n, added_bits, shift, lo, low_major)
+  kM31Block = 5     // M31 block twiddles: (log_n, inverse)
+};
+struct TableKey {
+  uint32_t kind, a, b, c, d, e;
+  bool operator<(const TableKey &o) const {
+    return std::tie(kind, a, b, c, d, e) < std::tie(o.kind, o.a, o.b, o.c, o.d, o.e);
+  }
+};
+struct TableEntry {
+  DevBuf buf[2];
+  size_t bytes = 0;
+  uint64_t last_use = 0;  // LRU clock
+  uint32_t pins = 0;      // open TablePins scopes / plans that hold it
+};
+
+class Context;
+// While a TablePins scope is open on a thread, every cached table the thread
+// obtains from `ctx` stays resident until the scope closes (its launches are
+// enqueued by then) or, after take(), until the new owner calls unpin.
+class TablePins {
+ public:
+  explicit TablePins(Context *ctx);
+  ~TablePins();
+  TablePins(const TablePins &) = delete;
+  TablePins &operator=(const TablePins &) = delete;
+  std::vector<TableKey> take();  // the pins move to the caller (plans keep theirs)
+  void add(const TableKey &k) { keys_.push_back(k); }
+  Context *ctx() const { return ctx_; }
+
+ private:
+  Context *ctx_;
+  TablePins *prev_;
+  std::vector<TableKey> keys_;
+};
+
 struct TuneKey {
   int field;
   uint32_t log_n, batch;
@@ -74,14 +115,18 @@ class Context {
   uint32_t mf_pos_log[2] = {0, 0};
   DevBuf bb_small_pair;    // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
   DevBuf mf_root[2];       // g_{2^11}^e, e < 2^11, Shoup pairs (tuned kernels)
-  std::map<std::tuple<int, uint32_t, uint32_t, int>, DevBuf> mf_twist;    // (field, lo, hi, inverse)
-  std::map<std::tuple<uint32_t, uint32_t, uint32_t>, DevBuf> bb_lde_mid;  // (L, a, shift_std)
-  std::map<std::pair<uint32_t, int>, DevBuf> m31_tw;                      // (L, inverse)
-  // (field, lo, hi, inverse, low_major), Montgomery words
-  std::map<std::tuple<int, uint32_t, uint32_t, int, int>, DevBuf> mf_twist_m;
-  std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, int>, std::pair<DevBuf, DevBuf>>
-      bb_lde_fast_tab;  // (L, a, shift, lo, low_major)
-  size_t table_bytes = 0;
+  // Per-shape tables (twists, LDE factors, M31 block twiddles): least recently
+  // used entries are freed when the cache would exceed table_limit bytes (an
+  // entry larger than the limit is still admitted alone).  Pinned entries are
+  // never evicted; an eviction waits for the device first, since kernels of
+  // any stream may still read an unpinned table.
+  std::map<TableKey, TableEntry> tables;
+  size_t table_limit = kDefaultTableLimit;
+  size_t cached_bytes = 0;  // bytes of `tables`
+  uint64_t table_clock = 0;
+  uint64_t evictions = 0;
+  static constexpr size_t kDefaultTableLimit = (size_t)4 << 30;
+  size_t table_bytes = 0;       // every device table the context holds
   std::vector<DevBuf> retired;  // outgrown position tables, freed with the context
 
   // scratch for the host-pointer API and V3 ping-pong
@@ -117,6 +162,15 @@ class Context {
   sfn_status acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **out);
   void release_lde_scratch(hipStream_t s);
   sfn_status upload_table(DevBuf &b, const void *host, size_t bytes);
+  void release_table(DevBuf &b);
+  // Cached table `k`: built by build(k, h0, h1) (host words of up to two buffers)
+  // on a miss; pinned in the innermost open TablePins scope of this thread.
+  using TableBuilder = void (*)(const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &h1);
+  sfn_status cached_table(const TableKey &k, TableBuilder build, const void **p0, const void **p1 = nullptr);
+  void unpin(const std::vector<TableKey> &keys);
+  // frees unpinned least-recently-used entries until `incoming` more bytes fit
+  void evict_for(size_t incoming);
+  sfn_status set_table_limit(size_t bytes);
   // field: SFN_FIELD_BABYBEAR or SFN_FIELD_KOALABEAR for the Montgomery-field tables
   sfn_status position_table(int field, uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);

@@ -2,15 +2,22 @@
 // the reference, which runs one synchronous call at a time).
 //
 //  - submit() validates and enqueues (FIFO), bounded by max_queue;
-//  - max_inflight worker threads each own one HIP stream; a free worker takes
-//    the queue head once its device-memory footprint fits the budget
-//    (admission by HBM footprint), plus queued jobs of the same shape
-//    (continuous batching): NTT jobs (any field) of one (field, dir, log_n)
-//    run as one multi-job launch per pass, LDE jobs of one shape run back to
-//    back on the worker's stream with a single completion wait;
-//  - host-memory jobs are staged through the worker's device buffers;
-//    device-memory jobs (device_ptrs = 1) run in place in HBM;
-//  - every job records submit / start / end times on the host steady clock.
+//  - max_inflight worker threads each own one HIP stream.  A free worker takes
+//    the oldest queued job it may run (worker 0 runs small jobs only when there
+//    are several workers, so small jobs never wait behind large ones), plus the
+//    queued jobs it can fuse with it (continuous batching):
+//      NTT jobs of any field, direction and size run together as one
+//      many-job launch per pass;
+//      LDE jobs of one (log_n, added_bits, shift) run as one batched LDE over
+//      their columns, laid out back to back in the worker's staging buffers
+//      (host-memory jobs are uploaded there; device-memory jobs are gathered
+//      there and scattered back by one segmented copy each);
+//  - admission by HBM footprint: every worker's staging buffers count against
+//    mem_budget_bytes for as long as it holds them; a job whose staging does
+//    not fit waits (or, for device-memory jobs, runs unfused in place);
+//  - every job records submit / start / end times on the host steady clock;
+//  - destroy fails the jobs still queued with SFN_ERR_BUSY, lets the running
+//    ones finish, and returns once no thread is inside sfn_server_wait.
 #include <string.h>
 
 #include <algorithm>
@@ -31,6 +38,11 @@ extern uint64_t sfn_now_ns();
 
 namespace {
 
+// jobs up to this many output elements are "small" (the dedicated worker's lane)
+constexpr uint64_t kSmallJobElems = 1ull << 22;
+// one fused group moves at most this many elements (bounds its latency)
+constexpr uint64_t kGroupElems = 1ull << 27;
+
 struct Job {
   uint64_t ticket = 0;
   sfn_request req{};
@@ -40,12 +52,21 @@ struct Job {
   sfn_status status = SFN_OK;
   bool done = false;
 
-  size_t footprint() const { return req.device_ptrs ? 0 : in_bytes + out_bytes; }
-  bool same_shape(const Job &o) const {
-    return req.kind == o.req.kind && req.field == o.req.field && req.dir == o.req.dir &&
-           req.log_n == o.req.log_n && req.added_bits == o.req.added_bits && req.shift_raw == o.req.shift_raw &&
-           req.device_ptrs == o.req.device_ptrs;
+  uint64_t out_elems() const { return out_bytes / 4; }
+  bool small() const { return out_elems() <= kSmallJobElems; }
+  // LDE jobs fuse when they share the transform; NTT jobs fuse with any NTT job
+  bool fuses_with(const Job &o) const {
+    if (req.kind != o.req.kind || req.device_ptrs != o.req.device_ptrs) return false;
+    if (req.kind == SFN_JOB_NTT) return req.log_n > 0 && o.req.log_n > 0;
+    return req.log_n == o.req.log_n && req.added_bits == o.req.added_bits && req.shift_raw == o.req.shift_raw;
   }
+};
+
+// A worker's device buffers: staging (in, out) for its current group, and the
+// descriptor arrays of its many-job launches and segmented copies.
+struct Staging {
+  sfn::DevBuf in, out, desc;
+  size_t held() const { return in.bytes + out.bytes; }
 };
 
 }  // namespace
@@ -58,105 +79,179 @@ struct sfn_server {
   std::deque<Job *> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
   uint64_t next_ticket = 1;
-  size_t reserved = 0;  // device bytes held by running host-memory jobs
+  size_t reserved = 0;  // device bytes of every worker's staging buffers (held or being allocated)
   uint32_t running = 0;
+  uint32_t waiting = 0;  // threads inside sfn_server_wait
   bool stop = false;
   std::vector<std::thread> workers;
 
   void worker_main(int idx);
-  sfn_status run_group(std::vector<Job *> &g, hipStream_t s, sfn::DevBuf &bin, sfn::DevBuf &bout,
-                       sfn::DevBuf &segs);
+  // picks the group under mu; returns false when there is nothing to run
+  bool pick(int idx, const Staging &stg, std::vector<Job *> &group, size_t &need_in, size_t &need_out, bool &staged);
+  sfn_status run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &st);
 };
 
 namespace {
 
-sfn_status grow(Context *c, sfn::DevBuf &b, size_t bytes) {
+sfn_status grow(sfn::DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes) return SFN_OK;
   if (b.ptr) (void)hipFree(b.ptr);
-  b.ptr = nullptr;
-  b.bytes = 0;
+  b = sfn::DevBuf{};
   if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
     (void)hipGetLastError();
     b.ptr = nullptr;
     return SFN_ERR_BUFFER_ALLOC_FAILED;
   }
   b.bytes = bytes;
-  (void)c;
   return SFN_OK;
+}
+
+void release(sfn::DevBuf &b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b = sfn::DevBuf{};
 }
 
 }  // namespace
 
-// Runs one group (same shape) on stream s and waits for it.
-sfn_status sfn_server::run_group(std::vector<Job *> &g, hipStream_t s, sfn::DevBuf &bin, sfn::DevBuf &bout,
-                                 sfn::DevBuf &segs) {
+// Under mu.  The head is the oldest queued job this worker may run (worker 0
+// takes small jobs only while other workers exist).  The group is the head
+// plus the queued jobs that fuse with it, in arrival order, within max_group,
+// max_group_cols (LDE columns) and kGroupElems.  Host-memory groups always
+// stage; a device-memory group of two or more LDE jobs stages when the budget
+// allows, else it runs in place one job after another.
+bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, size_t &need_in, size_t &need_out,
+                      bool &staged) {
+  const bool small_lane = idx == 0 && cfg.max_inflight > 1;
+  auto head_it = queue.begin();
+  while (head_it != queue.end() && small_lane && !(*head_it)->small()) ++head_it;
+  if (head_it == queue.end()) return false;
+  Job *head = *head_it;
+  const size_t others = reserved - stg.held();  // the other workers' staging
+  // staging grows to at least (in, out) and never shrinks while held
+  auto fits = [&](size_t in, size_t out) {
+    return others + std::max(stg.in.bytes, in) + std::max(stg.out.bytes, out) <= cfg.mem_budget_bytes;
+  };
+  const bool host = !head->req.device_ptrs;
+  if (host && !fits(head->in_bytes, head->out_bytes)) return false;  // wait for staging to free up
+  group.assign(1, head);
+  need_in = head->in_bytes;
+  need_out = head->out_bytes;
+  uint64_t elems = head->out_elems(), cols = head->req.batch;
+  for (auto it = std::next(head_it); it != queue.end() && group.size() < cfg.max_group; ++it) {
+    Job *j = *it;
+    if (!j->fuses_with(*head) || (small_lane && !j->small())) continue;
+    if (elems + j->out_elems() > kGroupElems) continue;
+    if (j->req.kind == SFN_JOB_LDE && cols + j->req.batch > cfg.max_group_cols) continue;
+    if (j->req.kind == SFN_JOB_LDE && !host && !fits(need_in + j->in_bytes, need_out + j->out_bytes)) continue;
+    if (host && !fits(need_in + j->in_bytes, need_out + j->out_bytes)) continue;
+    group.push_back(j);
+    need_in += j->in_bytes;
+    need_out += j->out_bytes;
+    elems += j->out_elems();
+    cols += j->req.batch;
+  }
+  staged = host || (head->req.kind == SFN_JOB_LDE && group.size() > 1);
+  for (Job *j : group) queue.erase(std::find(queue.begin(), queue.end(), j));
+  return true;
+}
+
+// Runs one group on stream s and waits for it.
+sfn_status sfn_server::run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &stg) {
   Context &c = *ctx;
+  sfn::TablePins pins(&c);
   const sfn_request &r0 = g[0]->req;
+  const bool host = !r0.device_ptrs;
   hipError_t e;
-  // device pointers of every member
+  // where every member's input and output live during the transform
   std::vector<const uint32_t *> din(g.size());
   std::vector<uint32_t *> dout(g.size());
-  if (r0.device_ptrs) {
-    for (size_t i = 0; i < g.size(); ++i) {
+  std::vector<sfn::CopySeg> gather, scatter;
+  uint64_t gather_total = 0, scatter_total = 0;
+  size_t oi = 0, oo = 0;
+  for (size_t i = 0; i < g.size(); ++i) {
+    if (!staged) {
       din[i] = g[i]->req.in;
       dout[i] = g[i]->req.out;
+      continue;
     }
-  } else {
-    size_t tin = 0, tout = 0;
-    for (Job *j : g) {
-      tin += j->in_bytes;
-      tout += j->out_bytes;
-    }
-    sfn_status st = grow(&c, bin, tin);
-    if (!st) st = grow(&c, bout, tout);
-    if (st) return st;
-    size_t oi = 0, oo = 0;
-    for (size_t i = 0; i < g.size(); ++i) {
-      din[i] = (const uint32_t *)((char *)bin.ptr + oi);
-      dout[i] = (uint32_t *)((char *)bout.ptr + oo);
+    din[i] = (const uint32_t *)((char *)stg.in.ptr + oi);
+    dout[i] = (uint32_t *)((char *)stg.out.ptr + oo);
+    if (host) {
       e = hipMemcpyAsync((void *)din[i], g[i]->req.in, g[i]->in_bytes, hipMemcpyHostToDevice, s);
       if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
-      oi += g[i]->in_bytes;
-      oo += g[i]->out_bytes;
+    } else {
+      gather.push_back(sfn::CopySeg{g[i]->req.in, (uint32_t *)din[i], gather_total});
+      gather_total += g[i]->in_bytes / 4;
+      scatter.push_back(sfn::CopySeg{dout[i], g[i]->req.out, scatter_total});
+      scatter_total += g[i]->out_bytes / 4;
     }
+    oi += g[i]->in_bytes;
+    oo += g[i]->out_bytes;
   }
-  if (r0.kind == SFN_JOB_LDE) {
-    for (size_t i = 0; i < g.size(); ++i) {
-      const sfn_request &r = g[i]->req;
-      sfn_status st = c.run_lde(r.log_n, r.batch, r.added_bits, r.shift_raw, SFN_LAYOUT_SOA, din[i], dout[i], s);
-      if (st) return st;
-    }
-  } else if (g.size() == 1 || r0.log_n == 0) {
-    for (size_t i = 0; i < g.size(); ++i) {
-      const sfn_request &r = g[i]->req;
-      sfn_status st = c.run_ntt(r.field, SFN_VARIANT_AUTO, r.dir, r.log_n, r.batch, SFN_LAYOUT_SOA, din[i], dout[i], s);
-      if (st) return st;
-    }
-  } else {
-    // one multi-job launch per pass
+  // device descriptor arrays of this group: [gather][scatter][many-job segments]
+  std::vector<sfn::MultiLaunch> ml;
+  const bool multi = r0.kind == SFN_JOB_NTT && g.size() > 1;
+  if (multi) {
     std::vector<sfn::MultiJob> mj;
     for (size_t i = 0; i < g.size(); ++i) {
       const sfn_request &r = g[i]->req;
       mj.push_back(sfn::MultiJob{r.field, r.dir, r.log_n, r.batch, din[i], dout[i]});
     }
-    std::vector<sfn::MultiLaunch> ml;
     sfn_status st = sfn::build_multi(&c, mj, ml);
     if (st) return st;
-    size_t total = 0;
-    for (auto &m : ml) total += m.segs.size() * sizeof(sfn::SegDesc);
-    if (grow(&c, segs, total)) return SFN_ERR_BUFFER_ALLOC_FAILED;
-    std::vector<sfn::SegDesc> flat;
-    for (auto &m : ml) flat.insert(flat.end(), m.segs.begin(), m.segs.end());
-    e = hipMemcpyAsync(segs.ptr, flat.data(), total, hipMemcpyHostToDevice, s);
+  }
+  const size_t cs = sizeof(sfn::CopySeg);
+  size_t nseg = 0;
+  for (auto &m : ml) nseg += m.segs.size();
+  std::vector<char> desc((gather.size() + scatter.size()) * cs + nseg * sizeof(sfn::SegDesc));
+  if (!gather.empty()) memcpy(desc.data(), gather.data(), gather.size() * cs);
+  if (!scatter.empty()) memcpy(desc.data() + gather.size() * cs, scatter.data(), scatter.size() * cs);
+  size_t off = (gather.size() + scatter.size()) * cs;
+  for (auto &m : ml) {
+    memcpy(desc.data() + off, m.segs.data(), m.segs.size() * sizeof(sfn::SegDesc));
+    off += m.segs.size() * sizeof(sfn::SegDesc);
+  }
+  if (!desc.empty()) {
+    if (grow(stg.desc, desc.size())) return SFN_ERR_BUFFER_ALLOC_FAILED;
+    e = hipMemcpyAsync(stg.desc.ptr, desc.data(), desc.size(), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
-    size_t off = 0;
+  }
+  char *dd = (char *)stg.desc.ptr;
+  if (!gather.empty()) {
+    e = sfn::launch_copy_segments((const sfn::CopySeg *)dd, (uint32_t)gather.size(), gather_total, s);
+    if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
+  }
+  if (r0.kind == SFN_JOB_LDE) {
+    if (staged) {  // the members' columns are back to back: one batched LDE
+      uint32_t cols = 0;
+      for (Job *j : g) cols += j->req.batch;
+      sfn_status st = c.run_lde(r0.log_n, cols, r0.added_bits, r0.shift_raw, SFN_LAYOUT_SOA, din[0], dout[0], s);
+      if (st) return st;
+    } else {
+      for (size_t i = 0; i < g.size(); ++i) {
+        const sfn_request &r = g[i]->req;
+        sfn_status st = c.run_lde(r.log_n, r.batch, r.added_bits, r.shift_raw, SFN_LAYOUT_SOA, din[i], dout[i], s);
+        if (st) return st;
+      }
+    }
+  } else if (!multi) {
+    const sfn_request &r = g[0]->req;
+    sfn_status st = c.run_ntt(r.field, SFN_VARIANT_AUTO, r.dir, r.log_n, r.batch, SFN_LAYOUT_SOA, din[0], dout[0], s);
+    if (st) return st;
+  } else {
+    size_t o = (gather.size() + scatter.size()) * cs;
     for (auto &m : ml) {
-      e = sfn::launch_multi_any(m, (const sfn::SegDesc *)segs.ptr + off, sfn::multi_grid(&c, m), s);
+      e = sfn::launch_multi_any(m, (const sfn::SegDesc *)(dd + o), sfn::multi_grid(&c, m), s);
       if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
-      off += m.segs.size();
+      o += m.segs.size() * sizeof(sfn::SegDesc);
     }
   }
-  if (!r0.device_ptrs) {
+  if (!scatter.empty()) {
+    e = sfn::launch_copy_segments((const sfn::CopySeg *)(dd + gather.size() * cs), (uint32_t)scatter.size(),
+                                  scatter_total, s);
+    if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
+  }
+  if (host) {
     for (size_t i = 0; i < g.size(); ++i) {
       e = hipMemcpyAsync(g[i]->req.out, dout[i], g[i]->out_bytes, hipMemcpyDeviceToHost, s);
       if (e != hipSuccess) return SFN_ERR_GPU_EXECUTION;
@@ -167,51 +262,60 @@ sfn_status sfn_server::run_group(std::vector<Job *> &g, hipStream_t s, sfn::DevB
 }
 
 void sfn_server::worker_main(int idx) {
-  (void)idx;
   (void)hipSetDevice(ctx->device);
   hipStream_t s = nullptr;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
-  sfn::DevBuf bin, bout, segs;
+  Staging stg;
+  size_t held = 0;  // this worker's share of `reserved` (its staging, allocated or about to be)
+  auto drop_staging = [&] {
+    release(stg.in);
+    release(stg.out);
+    release(stg.desc);
+    std::lock_guard<std::mutex> lk(mu);
+    reserved -= held;
+    held = 0;
+  };
   for (;;) {
     std::vector<Job *> group;
-    size_t foot = 0;
+    size_t need_in = 0, need_out = 0;
+    bool staged = false, shed = false;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] {
-        return stop || (!queue.empty() && reserved + queue.front()->footprint() <= cfg.mem_budget_bytes);
+        if (stop) return true;
+        if (pick(idx, stg, group, need_in, need_out, staged)) return true;
+        // an idle worker holding staging gives it back while jobs wait for budget
+        shed = held > 0 && !queue.empty();
+        return shed;
       });
-      if (stop && queue.empty()) break;
-      if (stop && !queue.empty() && reserved + queue.front()->footprint() > cfg.mem_budget_bytes) break;
-      Job *head = queue.front();
-      queue.pop_front();
-      group.push_back(head);
-      foot = head->footprint();
-      uint64_t cols = head->req.batch;
-      // continuous batching: take queued jobs of the same shape, small ones only
-      const size_t small = 16u << 20;
-      if (cfg.max_group > 1 && head->in_bytes + head->out_bytes <= small) {
-        for (auto it = queue.begin(); it != queue.end() && group.size() < cfg.max_group;) {
-          Job *j = *it;
-          if (j->same_shape(*head) && cols + j->req.batch <= cfg.max_group_cols &&
-              reserved + foot + j->footprint() <= cfg.mem_budget_bytes) {
-            group.push_back(j);
-            foot += j->footprint();
-            cols += j->req.batch;
-            it = queue.erase(it);
-          } else {
-            ++it;
-          }
+      if (group.empty() && !shed) break;  // stopping
+      if (!group.empty()) {
+        if (staged) {
+          const size_t grown = std::max(stg.in.bytes, need_in) + std::max(stg.out.bytes, need_out);
+          reserved += grown - held;
+          held = grown;
+        }
+        ++running;
+        const uint64_t t = sfn_now_ns();
+        for (Job *j : group) {
+          j->start_ns = t;
+          j->group_size = (uint32_t)group.size();
         }
       }
-      reserved += foot;
-      ++running;
-      const uint64_t t = sfn_now_ns();
-      for (Job *j : group) {
-        j->start_ns = t;
-        j->group_size = (uint32_t)group.size();
-      }
     }
-    sfn_status st = s ? run_group(group, s, bin, bout, segs) : SFN_ERR_GPU_EXECUTION;
+    if (shed) {
+      drop_staging();
+      cv_work.notify_all();
+      continue;
+    }
+    sfn_status st = SFN_ERR_GPU_EXECUTION;
+    if (s) {
+      st = SFN_OK;
+      if (staged && (grow(stg.in, need_in) || grow(stg.out, need_out))) st = SFN_ERR_BUFFER_ALLOC_FAILED;
+      if (!st) st = run_group(group, staged, s, stg);
+      if (st) (void)hipStreamSynchronize(s);
+    }
+    if (st) drop_staging();  // after a failure the worker starts again from nothing
     const uint64_t t = sfn_now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -220,15 +324,13 @@ void sfn_server::worker_main(int idx) {
         j->status = st;
         j->done = true;
       }
-      reserved -= foot;
       --running;
     }
     cv_done.notify_all();
     cv_work.notify_all();
   }
   if (s) (void)hipStreamSynchronize(s);
-  for (sfn::DevBuf *b : {&bin, &bout, &segs})
-    if (b->ptr) (void)hipFree(b->ptr);
+  drop_staging();
   if (s) (void)hipStreamDestroy(s);
 }
 
@@ -302,7 +404,8 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
   job->req = *req;
   job->in_bytes = in_b;
   job->out_bytes = out_b;
-  if (job->footprint() > srv->cfg.mem_budget_bytes) return SFN_ERR_BUFFER_ALLOC_FAILED;  // can never be admitted
+  // a host-memory job is staged whole: larger than the budget, it can never run
+  if (!req->device_ptrs && in_b + out_b > srv->cfg.mem_budget_bytes) return SFN_ERR_BUFFER_ALLOC_FAILED;
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->stop) return SFN_ERR_INVALID_ARGUMENT;
   if (srv->queue.size() >= srv->cfg.max_queue) return SFN_ERR_BUSY;
@@ -342,10 +445,13 @@ sfn_status sfn_server_wait(sfn_server *srv, uint64_t ticket, sfn_job_timing *tim
   auto it = srv->jobs.find(ticket);
   if (it == srv->jobs.end()) return SFN_ERR_INVALID_ARGUMENT;
   Job *j = it->second.get();
+  ++srv->waiting;
   srv->cv_done.wait(lk, [&] { return j->done; });
+  --srv->waiting;
   fill_timing(*j, timing);
   const sfn_status st = j->status;
   srv->jobs.erase(ticket);
+  if (srv->stop) srv->cv_done.notify_all();  // destroy may be waiting for the last waiter
   return st;
 }
 
@@ -361,9 +467,22 @@ void sfn_server_destroy(sfn_server *srv) {
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->stop = true;
+    // jobs that have not started will not run: their waiters get SFN_ERR_BUSY
+    const uint64_t t = sfn_now_ns();
+    for (Job *j : srv->queue) {
+      j->status = SFN_ERR_BUSY;
+      j->start_ns = j->end_ns = t;
+      j->done = true;
+    }
+    srv->queue.clear();
   }
   srv->cv_work.notify_all();
+  srv->cv_done.notify_all();
   for (auto &t : srv->workers) t.join();
+  {
+    std::unique_lock<std::mutex> lk(srv->mu);
+    srv->cv_done.wait(lk, [&] { return srv->waiting == 0; });
+  }
   delete srv;
 }
 

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sfn_synchronize": (ctypes.c_int, [vp, vp]),
     "sfn_twiddles_prepare": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32]),
     "sfn_twiddles_bytes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "sfn_twiddles_limit": (ctypes.c_int, [vp, ctypes.c_size_t]),
     "sfn_ntt_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_int, vp, vp, vp]),
     "sfn_coset_lde_batch_device": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32,

@@ -157,6 +157,10 @@ class HipContext:
         self.check(self._lib.sfn_twiddles_bytes(self.ptr, ctypes.byref(b)))
         return b.value
 
+    def twiddles_limit(self, max_bytes: int):
+        """Byte budget of the per-shape tables (0 = the default)."""
+        self.check(self._lib.sfn_twiddles_limit(self.ptr, max_bytes))
+
 
 _default_ctx = None
 _default_lock = threading.Lock()
