@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "sfntt", "libsfntt.so")
+LOADGEN = os.path.join(HERE, "sfntt", "serve_load")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -68,6 +69,13 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the serving benchmark's native load generator (links the library)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HERE, "serve_load.cpp"), "-L", os.path.dirname(LIB), "-lsfntt",
+           "-Wl,-rpath,$ORIGIN", "-o", LOADGEN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"load generator build failed: {' '.join(cmd)}\n{r.stderr}")
     return LIB
 
 

@@ -64,6 +64,7 @@ Context::~Context() {
   for (auto e : tev) (void)hipEventDestroy(e);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  chunks.release();
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1217,6 +1218,113 @@ static sfn_status host_call(Context &c, const uint32_t *in, size_t in_elems, uin
   return SFN_OK;
 }
 
+namespace sfn {
+
+// Column chunks of a host <-> device batch call: at least kChunkBytes of input
+// each.  SoA chunks are contiguous (up to 8); row-major chunks are column
+// windows moved by 2D copies, kept at >= 64 columns (256-byte row pieces, which
+// the copy engines move at full rate) and at most 4.
+constexpr size_t kChunkBytes = (size_t)32 << 20;
+
+uint32_t chunk_columns(uint64_t in_bytes, uint32_t batch, int layout) {
+  const bool rm = layout == SFN_LAYOUT_ROW_MAJOR;
+  uint64_t chunks = std::min<uint64_t>(rm ? 4 : 8, in_bytes / kChunkBytes);
+  if (rm) chunks = std::min<uint64_t>(chunks, batch / 64);
+  if (chunks <= 1) return 0;
+  uint32_t q = (uint32_t)((batch + chunks - 1) / chunks);
+  if (rm) q = (q + 15) & ~15u;  // whole tiles of the tuned kernels' lanes
+  return q < batch ? q : 0;
+}
+
+sfn_status run_chunked(Context &c, ChunkStreams &cs, hipStream_t after, const uint32_t *in, uint64_t in_rows,
+                       uint32_t *out, uint64_t out_rows, uint32_t batch, int layout, uint32_t q, uint32_t *d_in,
+                       uint32_t *d_out, const ChunkRun &run, double *kernel_ns) {
+  const bool rm = layout == SFN_LAYOUT_ROW_MAJOR;
+  const uint32_t chunks = (batch + q - 1) / q;
+  hipError_t e;
+  for (auto &x : cs.s)
+    if (!x && (e = hipStreamCreateWithFlags(&x, hipStreamNonBlocking)) != hipSuccess) {
+      x = nullptr;
+      return c.hip_fail(e, "transfer stream");
+    }
+  while (cs.ev.size() < 2 * (size_t)chunks + 1) {
+    hipEvent_t ev;
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return c.hip_fail(e, "event");
+    cs.ev.push_back(ev);
+  }
+  // work already queued on `after` (earlier calls) comes first
+  if (after) {
+    if ((e = hipEventRecord(cs.ev[2 * chunks], after)) != hipSuccess) return c.hip_fail(e, "event");
+    for (auto x : cs.s)
+      if (x != after && (e = hipStreamWaitEvent(x, cs.ev[2 * chunks], 0)) != hipSuccess)
+        return c.hip_fail(e, "stream order");
+  }
+  for (uint32_t k = 0; k < chunks; ++k) {
+    hipStream_t s = cs.s[k % ChunkStreams::kStreams];
+    const uint32_t c0 = k * q, cols = std::min(q, batch - c0);
+    uint32_t *ci = d_in + (size_t)c0 * in_rows, *co = d_out + (size_t)c0 * out_rows;
+    if (rm)
+      e = hipMemcpy2DAsync(ci, (size_t)cols * 4, in + c0, (size_t)batch * 4, (size_t)cols * 4, in_rows,
+                           hipMemcpyHostToDevice, s);
+    else
+      e = hipMemcpyAsync(ci, in + (size_t)c0 * in_rows, (size_t)cols * in_rows * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return c.hip_fail(e, "upload");
+    if ((e = hipEventRecord(cs.ev[2 * k], s)) != hipSuccess) return c.hip_fail(e, "event");
+    sfn_status st = run(cols, ci, co, s);
+    if (st) return st;
+    if ((e = hipEventRecord(cs.ev[2 * k + 1], s)) != hipSuccess) return c.hip_fail(e, "event");
+    if (rm)
+      e = hipMemcpy2DAsync(out + c0, (size_t)batch * 4, co, (size_t)cols * 4, (size_t)cols * 4, out_rows,
+                           hipMemcpyDeviceToHost, s);
+    else
+      e = hipMemcpyAsync(out + (size_t)c0 * out_rows, co, (size_t)cols * out_rows * 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return c.hip_fail(e, "download");
+  }
+  for (auto x : cs.s)
+    if ((e = hipStreamSynchronize(x)) != hipSuccess) return c.hip_fail(e, "GPU execution");
+  double ns = 0;
+  for (uint32_t k = 0; k < chunks; ++k) {
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, cs.ev[2 * k], cs.ev[2 * k + 1])) != hipSuccess)
+      return c.hip_fail(e, "event timing");
+    ns += (double)ms * 1e6;
+  }
+  if (kernel_ns) *kernel_ns = ns;
+  return SFN_OK;
+}
+
+}  // namespace sfn
+
+// The same call split into column chunks that run on the context's transfer
+// streams in turn, so the upload of one chunk, the kernels of another and the
+// download of a third overlap (the two directions of the PCIe link run at
+// once).  run(cols, d_in, d_out, stream) transforms `cols` columns.  device_ns
+// is the sum of the chunks' kernel spans.  Calls that do not split take host_call.
+template <class Run>
+static sfn_status host_call_chunked(Context &c, const uint32_t *in, uint64_t in_rows, uint32_t *out, uint64_t out_rows,
+                                    uint32_t batch, int layout, bool in_place_ok, uint64_t *device_ns, Run run) {
+  const size_t in_elems = (size_t)batch * in_rows, out_elems = (size_t)batch * out_rows;
+  const uint32_t q = sfn::chunk_columns(in_elems * 4, batch, layout);
+  if (!q)
+    return host_call(c, in, in_elems, out, out_elems, in_place_ok, device_ns,
+                     [&](const uint32_t *di, uint32_t *dout) { return run(batch, di, dout, c.stream); });
+  (void)hipSetDevice(c.device);
+  sfn_status st = c.ensure(c.scratch_in, in_elems * 4);
+  if (st) return st;
+  uint32_t *d_in = (uint32_t *)c.scratch_in.ptr;
+  uint32_t *d_out = d_in;
+  if (!in_place_ok || out_elems != in_elems) {
+    if ((st = c.ensure(c.scratch_out, out_elems * 4))) return st;
+    d_out = (uint32_t *)c.scratch_out.ptr;
+  }
+  double ns = 0;
+  st = sfn::run_chunked(c, c.chunks, c.stream, in, in_rows, out, out_rows, batch, layout, q, d_in, d_out, run, &ns);
+  if (st) return st;
+  if (device_ns) *device_ns = (uint64_t)ns;
+  c.timing_pending = false;  // the context stream's event pair does not bracket this call
+  return SFN_OK;
+}
+
 extern "C" {
 
 const char *sfn_status_string(sfn_status s) {
@@ -1311,9 +1419,14 @@ sfn_status sfn_ntt(sfn_ctx *ctx, sfn_field field, sfn_variant variant, sfn_direc
     if (device_ns) *device_ns = 0;
     return SFN_OK;
   }
-  return host_call(c, in, elems, out, elems, true, device_ns, [&](const uint32_t *di, uint32_t *dout) {
-    return c.run_ntt(field, variant, dir, log_n, batch, layout, di, dout, c.stream);
-  });
+  if (variant == SFN_STOCKHAM_R2)  // its ping-pong array is one per context: one stream
+    return host_call(c, in, elems, out, elems, true, device_ns, [&](const uint32_t *di, uint32_t *dout) {
+      return c.run_ntt(field, variant, dir, log_n, batch, layout, di, dout, c.stream);
+    });
+  return host_call_chunked(c, in, 1ull << log_n, out, 1ull << log_n, batch, layout, true, device_ns,
+                           [&](uint32_t cols, const uint32_t *di, uint32_t *dout, hipStream_t s) {
+                             return c.run_ntt(field, variant, dir, log_n, cols, layout, di, dout, s);
+                           });
 }
 
 sfn_status sfn_coset_lde_batch(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
@@ -1324,10 +1437,10 @@ sfn_status sfn_coset_lde_batch(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uin
   sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
   if (st) return st;
   if (!in || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
-  const size_t ni = (size_t)batch << log_n, no = (size_t)batch << (log_n + added_bits);
-  return host_call(c, in, ni, out, no, false, device_ns, [&](const uint32_t *di, uint32_t *dout) {
-    return c.run_lde(log_n, batch, added_bits, shift_raw, layout, di, dout, c.stream);
-  });
+  return host_call_chunked(c, in, 1ull << log_n, out, 1ull << (log_n + added_bits), batch, layout, false, device_ns,
+                           [&](uint32_t cols, const uint32_t *di, uint32_t *dout, hipStream_t s) {
+                             return c.run_lde(log_n, cols, added_bits, shift_raw, layout, di, dout, s);
+                           });
 }
 
 sfn_status sfn_coset_lde_natural(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
@@ -1338,10 +1451,10 @@ sfn_status sfn_coset_lde_natural(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, u
   sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
   if (st) return st;
   if (!in || !out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
-  const size_t ni = (size_t)batch << log_n, no = (size_t)batch << (log_n + added_bits);
-  return host_call(c, in, ni, out, no, false, device_ns, [&](const uint32_t *di, uint32_t *dout) {
-    return c.run_lde(log_n, batch, added_bits, shift_raw, layout, di, dout, c.stream, nullptr, true);
-  });
+  return host_call_chunked(c, in, 1ull << log_n, out, 1ull << (log_n + added_bits), batch, layout, false, device_ns,
+                           [&](uint32_t cols, const uint32_t *di, uint32_t *dout, hipStream_t s) {
+                             return c.run_lde(log_n, cols, added_bits, shift_raw, layout, di, dout, s, nullptr, true);
+                           });
 }
 
 sfn_status sfn_pointwise_mul(sfn_ctx *ctx, sfn_field field, const uint32_t *a, const uint32_t *b, uint32_t *out,

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -64,6 +65,23 @@ struct TableEntry {
   uint32_t pins = 0;      // open TablePins scopes / plans that hold it
 };
 
+// Streams and events of a host <-> device call split into column chunks
+// (created on first use; the context and every server worker own one set).
+struct ChunkStreams {
+  static constexpr int kStreams = 3;
+  hipStream_t s[kStreams] = {nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> ev;  // a start / end pair per chunk, and one ordering event
+  void release() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    for (auto &x : s)
+      if (x) {
+        (void)hipStreamDestroy(x);
+        x = nullptr;
+      }
+  }
+};
+
 class Context;
 // While a TablePins scope is open on a thread, every cached table the thread
 // obtains from `ctx` stays resident until the scope closes (its launches are
@@ -97,6 +115,7 @@ class Context {
  public:
   int device = 0;
   hipStream_t stream = nullptr;
+  ChunkStreams chunks;  // host-pointer calls split into column chunks
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // (ev0, ev1) bracket the last launch sequence on `stream` (read by sfn_last_device_ns)
   bool timing_pending = false;
@@ -234,6 +253,18 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs, std::vec
 uint32_t multi_grid(const Context *ctx, const MultiLaunch &L);
 hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_t grid, hipStream_t s);
 sfn_status validate_lde(Context *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits, int layout);
+
+// Host <-> device batch calls in column chunks: chunk_columns gives the
+// columns per chunk for `in_bytes` of input (0: one piece is better), and
+// run_chunked uploads chunk k, runs run(cols, d_in_k, d_out_k, stream) and
+// downloads it on stream k mod 3 of `cs`, after the work already queued on
+// `after` (may be null).  d_in / d_out hold the whole batch (chunk k at column
+// k * q; row-major chunks compact).  kernel_ns: the sum of the chunks' spans.
+using ChunkRun = std::function<sfn_status(uint32_t cols, const uint32_t *d_in, uint32_t *d_out, hipStream_t s)>;
+uint32_t chunk_columns(uint64_t in_bytes, uint32_t batch, int layout);
+sfn_status run_chunked(Context &c, ChunkStreams &cs, hipStream_t after, const uint32_t *in, uint64_t in_rows,
+                       uint32_t *out, uint64_t out_rows, uint32_t batch, int layout, uint32_t q, uint32_t *d_in,
+                       uint32_t *d_out, const ChunkRun &run, double *kernel_ns);
 // [a, a + a_bytes) and [b, b + b_bytes) share a byte
 bool ranges_overlap(const void *a, size_t a_bytes, const void *b, size_t b_bytes);
 

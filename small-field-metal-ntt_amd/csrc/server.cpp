@@ -3,15 +3,18 @@
 //
 //  - submit() validates and enqueues (FIFO), bounded by max_queue;
 //  - max_inflight worker threads each own one HIP stream.  A free worker takes
-//    the oldest queued job it may run (worker 0 runs small jobs only when there
-//    are several workers, so small jobs never wait behind large ones), plus the
-//    queued jobs it can fuse with it (continuous batching):
+//    the oldest queued job it may run, plus the queued jobs it can fuse with
+//    it (continuous batching).  Large jobs (more than kSmallJobElems output
+//    elements) run one at a time, as they fill the device alone; worker 0 runs
+//    small jobs only when there are several workers, so small jobs never wait
+//    behind large ones.  Fusion:
 //      NTT jobs of any field, direction and size run together as one
 //      many-job launch per pass;
 //      LDE jobs of one (log_n, added_bits, shift) run as one batched LDE over
 //      their columns, laid out back to back in the worker's staging buffers
-//      (host-memory jobs are uploaded there; device-memory jobs are gathered
-//      there and scattered back by one segmented copy each);
+//      (host-memory jobs are uploaded there; small device-memory jobs are
+//      gathered there and scattered back by one segmented copy each, which
+//      saves launches where launches dominate);
 //  - admission by HBM footprint: every worker's staging buffers count against
 //    mem_budget_bytes for as long as it holds them; a job whose staging does
 //    not fit waits (or, for device-memory jobs, runs unfused in place);
@@ -81,14 +84,15 @@ struct sfn_server {
   uint64_t next_ticket = 1;
   size_t reserved = 0;  // device bytes of every worker's staging buffers (held or being allocated)
   uint32_t running = 0;
-  uint32_t waiting = 0;  // threads inside sfn_server_wait
+  bool large_running = false;  // a group of large jobs is on the device
+  uint32_t waiting = 0;        // threads inside sfn_server_wait
   bool stop = false;
   std::vector<std::thread> workers;
 
   void worker_main(int idx);
   // picks the group under mu; returns false when there is nothing to run
   bool pick(int idx, const Staging &stg, std::vector<Job *> &group, size_t &need_in, size_t &need_out, bool &staged);
-  sfn_status run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &st);
+  sfn_status run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &st, sfn::ChunkStreams &cs);
 };
 
 namespace {
@@ -122,8 +126,10 @@ void release(sfn::DevBuf &b) {
 bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, size_t &need_in, size_t &need_out,
                       bool &staged) {
   const bool small_lane = idx == 0 && cfg.max_inflight > 1;
+  // the oldest job this worker may start: small ones always, a large one when
+  // this is not the small lane and no large group is running
   auto head_it = queue.begin();
-  while (head_it != queue.end() && small_lane && !(*head_it)->small()) ++head_it;
+  while (head_it != queue.end() && !(*head_it)->small() && (small_lane || large_running)) ++head_it;
   if (head_it == queue.end()) return false;
   Job *head = *head_it;
   const size_t others = reserved - stg.held();  // the other workers' staging
@@ -139,7 +145,9 @@ bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, si
   uint64_t elems = head->out_elems(), cols = head->req.batch;
   for (auto it = std::next(head_it); it != queue.end() && group.size() < cfg.max_group; ++it) {
     Job *j = *it;
-    if (!j->fuses_with(*head) || (small_lane && !j->small())) continue;
+    if (!j->fuses_with(*head) || j->small() != head->small()) continue;
+    // device-memory LDE jobs are gathered into staging only while launches dominate
+    if (j->req.kind == SFN_JOB_LDE && !host && !head->small()) continue;
     if (elems + j->out_elems() > kGroupElems) continue;
     if (j->req.kind == SFN_JOB_LDE && cols + j->req.batch > cfg.max_group_cols) continue;
     if (j->req.kind == SFN_JOB_LDE && !host && !fits(need_in + j->in_bytes, need_out + j->out_bytes)) continue;
@@ -156,12 +164,30 @@ bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, si
 }
 
 // Runs one group on stream s and waits for it.
-sfn_status sfn_server::run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &stg) {
+sfn_status sfn_server::run_group(std::vector<Job *> &g, bool staged, hipStream_t s, Staging &stg,
+                                 sfn::ChunkStreams &xfer) {
   Context &c = *ctx;
   sfn::TablePins pins(&c);
   const sfn_request &r0 = g[0]->req;
   const bool host = !r0.device_ptrs;
   hipError_t e;
+  if (host && g.size() == 1 && !g[0]->small()) {
+    // one large host-memory job: column chunks on the worker's transfer
+    // streams, so its upload, kernels and download overlap (as the plain
+    // host-pointer calls do)
+    const uint64_t in_rows = 1ull << r0.log_n;
+    const uint64_t out_rows = r0.kind == SFN_JOB_LDE ? in_rows << r0.added_bits : in_rows;
+    const uint32_t q = sfn::chunk_columns(g[0]->in_bytes, r0.batch, SFN_LAYOUT_SOA);
+    if (q) {
+      const sfn::ChunkRun run = [&](uint32_t cols, const uint32_t *di, uint32_t *dout, hipStream_t x) {
+        return r0.kind == SFN_JOB_LDE
+                   ? c.run_lde(r0.log_n, cols, r0.added_bits, r0.shift_raw, SFN_LAYOUT_SOA, di, dout, x)
+                   : c.run_ntt(r0.field, SFN_VARIANT_AUTO, r0.dir, r0.log_n, cols, SFN_LAYOUT_SOA, di, dout, x);
+      };
+      return sfn::run_chunked(c, xfer, s, r0.in, in_rows, r0.out, out_rows, r0.batch, SFN_LAYOUT_SOA, q,
+                              (uint32_t *)stg.in.ptr, (uint32_t *)stg.out.ptr, run, nullptr);
+    }
+  }
   // where every member's input and output live during the transform
   std::vector<const uint32_t *> din(g.size());
   std::vector<uint32_t *> dout(g.size());
@@ -266,6 +292,7 @@ void sfn_server::worker_main(int idx) {
   hipStream_t s = nullptr;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
   Staging stg;
+  sfn::ChunkStreams cs;  // large host-memory jobs run in column chunks on these
   size_t held = 0;  // this worker's share of `reserved` (its staging, allocated or about to be)
   auto drop_staging = [&] {
     release(stg.in);
@@ -278,7 +305,7 @@ void sfn_server::worker_main(int idx) {
   for (;;) {
     std::vector<Job *> group;
     size_t need_in = 0, need_out = 0;
-    bool staged = false, shed = false;
+    bool staged = false, shed = false, large = false;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] {
@@ -296,6 +323,8 @@ void sfn_server::worker_main(int idx) {
           held = grown;
         }
         ++running;
+        large = !group[0]->small();
+        if (large) large_running = true;
         const uint64_t t = sfn_now_ns();
         for (Job *j : group) {
           j->start_ns = t;
@@ -312,8 +341,12 @@ void sfn_server::worker_main(int idx) {
     if (s) {
       st = SFN_OK;
       if (staged && (grow(stg.in, need_in) || grow(stg.out, need_out))) st = SFN_ERR_BUFFER_ALLOC_FAILED;
-      if (!st) st = run_group(group, staged, s, stg);
-      if (st) (void)hipStreamSynchronize(s);
+      if (!st) st = run_group(group, staged, s, stg, cs);
+      if (st) {
+        (void)hipStreamSynchronize(s);
+        for (auto x : cs.s)
+          if (x) (void)hipStreamSynchronize(x);
+      }
     }
     if (st) drop_staging();  // after a failure the worker starts again from nothing
     const uint64_t t = sfn_now_ns();
@@ -324,13 +357,17 @@ void sfn_server::worker_main(int idx) {
         j->status = st;
         j->done = true;
       }
+      if (large) large_running = false;
       --running;
     }
     cv_done.notify_all();
     cv_work.notify_all();
   }
   if (s) (void)hipStreamSynchronize(s);
+  for (auto x : cs.s)
+    if (x) (void)hipStreamSynchronize(x);
   drop_staging();
+  cs.release();
   if (s) (void)hipStreamDestroy(s);
 }
 

@@ -147,6 +147,8 @@ hipError_t hipHostFree(void *p);
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind k, hipStream_t s);
 hipError_t hipMemsetAsync(void *dst, int v, size_t bytes, hipStream_t s);
+hipError_t hipMemcpy2DAsync(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                            hipMemcpyKind k, hipStream_t s);
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);

@@ -395,6 +395,22 @@ hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKin
     return hipSuccess;
   });
 }
+hipError_t hipMemcpy2DAsync(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                            hipMemcpyKind k, hipStream_t s) {
+  if (inject("memcpy")) return hipErrorInvalidValue;
+  if (width > dpitch || width > spitch) return hipErrorInvalidValue;
+  const size_t dspan = height ? (height - 1) * dpitch + width : 0, sspan = height ? (height - 1) * spitch + width : 0;
+  if ((k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToDevice) && dspan && !known_device(dst, dspan))
+    return hipErrorInvalidValue;
+  if ((k == hipMemcpyDeviceToHost || k == hipMemcpyDeviceToDevice) && sspan && !known_device(src, sspan))
+    return hipErrorInvalidValue;
+  return run_or_record(s, [=]() {
+    for (size_t r = 0; r < height; ++r)
+      memmove((char *)dst + r * dpitch, (const char *)src + r * spitch, width);
+    return hipSuccess;
+  });
+}
+
 hipError_t hipMemsetAsync(void *dst, int v, size_t bytes, hipStream_t s) {
   if (bytes && !known_device(dst, bytes)) return hipErrorInvalidValue;
   return run_or_record(s, [=]() {

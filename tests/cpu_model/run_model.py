@@ -114,6 +114,20 @@ def suite_lde(ctx):
             same(y, exp, f"lde soa 2^{log_n}x{batch} a={a}")
             y, _ = ctx.coset_lde(to_rm(x, log_n, batch), log_n, batch, a, shift, L.LAYOUT_ROW_MAJOR)
             same(from_rm(y, log_n + a, batch), exp, f"lde row-major 2^{log_n}x{batch} a={a}")
+    # host-pointer calls of more than one 32 MiB chunk run as column chunks on
+    # several streams (SoA: contiguous; row-major: 2D copies of column windows)
+    for log_n, batch, a in ((13, 2200, 1),):
+        x = orc.lcg_fill(77 + log_n, batch << log_n, BB)
+        exp = orc.coset_lde_batch(x, log_n, batch, a)
+        log(f"[lde] chunked host call 2^{log_n}x{batch} a={a}")
+        y, _ = ctx.coset_lde(x, log_n, batch, a)
+        same(y, exp, f"chunked lde soa 2^{log_n}x{batch}")
+        y, _ = ctx.coset_lde(to_rm(x, log_n, batch), log_n, batch, a, 0, L.LAYOUT_ROW_MAJOR)
+        same(from_rm(y, log_n + a, batch), exp, f"chunked lde row-major 2^{log_n}x{batch}")
+    x = orc.lcg_fill(5, 2200 << 13, KB)
+    log("[lde] chunked host NTT 2^13x2200 inverse")
+    y, _ = ctx.ntt(KB, L.VARIANT_AUTO, 1, 13, 2200, to_rm(x, 13, 2200), L.LAYOUT_ROW_MAJOR)
+    same(from_rm(y, 13, 2200), orc.ntt_batch(KB, True, 13, 2200, x), "chunked ntt")
     import closed_forms as cf
     for log_n, batch, a in ((6, 5, 2), (12, 8, 1), (13, 16, 1), (10, 4, 0)):
         x = orc.lcg_fill(4242 + log_n, batch << log_n, BB)
@@ -301,6 +315,11 @@ def suite_server(ctx):
         groups = [g for *_, g in [(r[0], r[1]) for r in res]]
         check(max(groups) > 1, "no job was fused")
         log(f"[server] largest group {max(groups)}, mean {np.mean(groups):.2f}")
+    # a large host-memory job runs in column chunks on the worker's transfer streams
+    log("[server] large host-memory LDE (column chunks)")
+    for st, g, shape, exp, got in _serve(ctx, [("lde", BB, 14, 1100, 1), ("ntt", M31, 10, 1, 0)], 0, inflight=2):
+        check(st == 0, f"job status {st}")
+        same(got, exp, f"large host job {shape}")
     # admission by footprint: host jobs of 1 MiB in + 2 MiB out under a 7 MiB budget
     base = _held(ctx)
     lib.sfn_model_reset_counters()

@@ -16,6 +16,11 @@ digests.json   SHA-256 of the full outputs of the benchmark workloads, over
                                  512 uneven columns (SplitMix64 seed 0x5EED0004:
                                  field = draw1 & 1, log_n = 10 + draw2 % 13, LCG
                                  seed = draw3), forward, outputs concatenated
+                 serving_slice_<k>
+                                 job k of a 20-job slice of the serving stream at
+                                 its stated sizes (serving_slice() below): a 2^20 x
+                                 256 LDE at 4x and one at 2x, an M31 2^22 NTT, and
+                                 17 jobs drawn from the stream's distribution
 Inputs come from the oracle's restatement of the reference LCG; every output is
 the oracle's.  The files are data only (numpy arrays without pickling, JSON).
 """
@@ -47,6 +52,33 @@ def uneven_shapes(seed=0x5EED0004, count=512):
         a, b, c = next(it), next(it), next(it)
         out.append((int(a & 1), 10 + int(b % 13), int(c)))
     return out
+
+
+def serving_slice():
+    """(kind, log_n, batch, added_bits, seed) of 20 jobs of the serving stream
+    (60% BabyBear coset LDE 2^14..2^20 x 8..256 columns at 2x/4x with the
+    reference shift, 40% M31 forward NTT 2^12..2^22): the three largest shapes,
+    then 17 draws from the same distribution with LDE jobs capped at 2^17 x 64
+    so the slice's outputs stay near 7 GiB.  Job k's input is the reference LCG
+    with seed 0x5EED0005 + k (BabyBear Montgomery form for an LDE, M31 for an NTT)."""
+    rng = np.random.default_rng(0x5EED0005)
+    jobs = [("lde", 20, 256, 2), ("lde", 20, 256, 1), ("ntt", 22, 1, 0)]
+    while len(jobs) < 20:
+        if rng.random() < 0.6:
+            jobs.append(("lde", int(rng.integers(14, 18)), int(rng.integers(8, 65)), int(rng.integers(1, 3))))
+        else:
+            jobs.append(("ntt", int(rng.integers(12, 23)), 1, 0))
+    return [(k, n, b, a, 0x5EED0005 + i) for i, (k, n, b, a) in enumerate(jobs)]
+
+
+def serving_slice_input(kind, log_n, batch, seed):
+    return orc.lcg_fill(seed, batch << log_n, BB if kind == "lde" else M31)
+
+
+def serving_slice_output(kind, log_n, batch, added_bits, x):
+    if kind == "lde":
+        return orc.coset_lde_batch(x, log_n, batch, added_bits)
+    return orc.m31_forward(x)
 
 
 def small_fixtures() -> dict:
@@ -85,6 +117,9 @@ def digests() -> dict:
         x = orc.lcg_fill(seed, 1 << ln, field)
         h.update(np.ascontiguousarray(orc.ntt_batch(field, False, ln, 1, x), dtype="<u4").tobytes())
     d["uneven_512_columns_forward"] = h.hexdigest()
+    for k, (kind, log_n, batch, a, seed) in enumerate(serving_slice()):
+        x = serving_slice_input(kind, log_n, batch, seed)
+        d[f"serving_slice_{k}"] = sha(serving_slice_output(kind, log_n, batch, a, x))
     return d
 
 

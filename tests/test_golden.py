@@ -57,3 +57,17 @@ def test_digests_of_single_column_transforms(digests):
     assert sha(orc.bb_forward(orc.lcg_fill(12345 + n, n, BB))) == digests["bb_fwd_2e16"]
     n = 1 << 20
     assert sha(orc.m31_forward(orc.lcg_fill(12345 + n, n, M31))) == digests["m31_fwd_2e20"]
+
+
+def test_digests_of_small_serving_slice_jobs(digests):
+    # the slice jobs of at most 2^21 output elements, recomputed by the oracle
+    from golden import make_golden as mg
+
+    checked = 0
+    for k, (kind, log_n, batch, a, seed) in enumerate(mg.serving_slice()):
+        if (batch << (log_n + a)) > (1 << 21):
+            continue
+        x = mg.serving_slice_input(kind, log_n, batch, seed)
+        assert sha(mg.serving_slice_output(kind, log_n, batch, a, x)) == digests[f"serving_slice_{k}"], k
+        checked += 1
+    assert checked >= 5

@@ -343,3 +343,81 @@ def test_serving_admission_and_errors(ctx):
     assert ctx._lib.sfn_server_submit(srv, ctypes.byref(bad2), ctypes.byref(t)) == L.ERR_INVALID_SIZE
     assert ctx._lib.sfn_server_poll(srv, 12345, None) == L.ERR_INVALID_ARGUMENT
     ctx._lib.sfn_server_destroy(srv)
+
+
+def test_table_cache_stays_within_its_budget():
+    # Every LDE shift has its own post / point-factor tables: 30 distinct shifts
+    # under a 1 MiB budget keep the cache bounded, results stay exact, and a
+    # plan's tables survive evictions until the plan is destroyed
+    # (the reference's TwiddleCache keeps one log_n, src/ntt/twiddle_cache.rs:29-82).
+    c = sfntt.HipContext(0)
+    x = orc.lcg_fill(3, 8 << 12, orc.FIELD_BB)
+    c.coset_lde(x, 12, 8, 1, orc.bb_to_monty(3))
+    base = c.twiddles_bytes()
+    c.twiddles_limit(1 << 20)
+    for k in range(30):
+        shift = orc.bb_to_monty(5 + 2 * k)
+        y, _ = c.coset_lde(x, 12, 8, 1, shift)
+        assert np.array_equal(y, orc.coset_lde_batch(x, 12, 8, 1, shift)), k
+        assert c.twiddles_bytes() <= base + (1 << 20), (k, c.twiddles_bytes())
+    specs = [(0, 13, 11), (1, 14, 12), (0, 18, 13), (1, 16, 14)]
+    bufs, descs, xs = [], [], []
+    for f, log_n, sd in specs:
+        xx = synth.field_data(f, sd, 1 << log_n)
+        b = sfntt.DeviceBuffer(c, xx.size)
+        b.upload(xx)
+        bufs.append(b)
+        xs.append(xx)
+        descs.append(L.JobDesc(f, L.FORWARD, log_n, 1, b.ptr, b.ptr))
+    arr = (L.JobDesc * len(descs))(*descs)
+    plan = ctypes.c_void_p()
+    c.check(c._lib.sfn_plan_create(c.ptr, arr, len(descs), 1, ctypes.byref(plan)))
+    c.twiddles_limit(1)
+    for k in range(3):
+        c.coset_lde(x, 12, 8, 1, orc.bb_to_monty(101 + k))
+    c.check(c._lib.sfn_plan_launch(plan, None))
+    c.check(c._lib.sfn_synchronize(c.ptr, None))
+    for (f, log_n, sd), b, xx in zip(specs, bufs, xs):
+        assert np.array_equal(b.download(), orc.ntt_batch(f, False, log_n, 1, xx)), (f, log_n)
+    c._lib.sfn_plan_destroy(plan)
+    c.twiddles_limit(0)
+    for b in bufs:
+        b.free()
+    c.close()
+
+
+def test_server_fuses_jobs_of_every_ntt_shape_and_same_shape_ldes(ctx):
+    # queued NTT jobs of different fields, sizes and directions run as one
+    # many-job launch set; queued LDE jobs of one shape run as one batched LDE
+    # (host memory: staged back to back; device memory: gathered and scattered)
+    srv = make_server(ctx, inflight=1)
+    keep = []
+    for dev in (0, 1):
+        for i in range(40):
+            if i % 2:
+                kind, f, log_n, batch, a, d = "lde", 1, 11, 1 + i % 5, 1, 0
+            else:
+                kind, f, log_n, batch, a, d = "ntt", (0, 1, 2)[i % 3], 8 + i % 9, 1, 0, (i // 2) % 2
+            x = synth.field_data(f, 5000 + i + 100 * dev, batch << log_n)
+            nout = batch << (log_n + a)
+            if dev:
+                din, dout = sfntt.DeviceBuffer(ctx, x.size), sfntt.DeviceBuffer(ctx, nout)
+                din.upload(x)
+                inp, outp, out = din.ptr, dout.ptr, dout
+            else:
+                out = np.zeros(nout, np.uint32)
+                inp, outp, din = x.ctypes.data, out.ctypes.data, None
+            r = L.Request(L.JOB_LDE if kind == "lde" else L.JOB_NTT, f, d, log_n, batch, a, 0, dev, inp, outp)
+            t = ctypes.c_uint64()
+            ctx.check(ctx._lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+            keep.append((t.value, kind, f, log_n, batch, a, d, x, out, dev, din))
+    groups = []
+    for t, kind, f, log_n, batch, a, d, x, out, dev, din in keep:
+        tm = L.JobTiming()
+        ctx.check(ctx._lib.sfn_server_wait(srv, t, ctypes.byref(tm)))
+        groups.append(tm.group_size)
+        got = out.download() if dev else out
+        exp = orc.coset_lde_batch(x, log_n, batch, a) if kind == "lde" else orc.ntt_batch(f, d == 1, log_n, batch, x)
+        assert np.array_equal(got, exp), (kind, f, log_n, batch, d, dev)
+    ctx._lib.sfn_server_destroy(srv)
+    assert max(groups) > 1

@@ -127,3 +127,46 @@ def test_uneven_512_columns_plan_digest(ctx, digests):
         ctx._lib.sfn_plan_destroy(plan)
         for b in bufs:
             b.free()
+
+
+def test_serving_slice_at_stated_sizes(ctx, digests):
+    # 20 jobs of the serving stream at its stated sizes (a 2^20 x 256 LDE at 4x
+    # and at 2x, an M31 2^22 NTT, 17 drawn jobs) submitted to one server at
+    # once, even jobs from host memory and odd jobs from device memory; every
+    # output against the oracle's digest (src/lde/mod.rs:194-271 compares the
+    # GPU LDE with the CPU one the same way, per job)
+    from golden import make_golden as mg
+
+    lib = ctx._lib
+    cfg = L.ServerConfig(0, 4, 0, 0, 0)
+    srv = ctypes.c_void_p()
+    ctx.check(lib.sfn_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(srv)))
+    keep = []
+    try:
+        for k, (kind, log_n, batch, a, seed) in enumerate(mg.serving_slice()):
+            x = mg.serving_slice_input(kind, log_n, batch, seed)
+            nout = batch << (log_n + a)
+            dev = k % 2
+            print(f"[golden] serving slice job {k}: {kind} 2^{log_n} x {batch} a={a} device_ptrs={dev}", flush=True)
+            if dev:
+                din, dout = sfntt.DeviceBuffer(ctx, x.size), sfntt.DeviceBuffer(ctx, nout)
+                din.upload(x)
+                inp, outp, out = din.ptr, dout.ptr, dout
+            else:
+                out = np.zeros(nout, np.uint32)
+                inp, outp, din = x.ctypes.data, out.ctypes.data, x
+            r = L.Request(L.JOB_LDE if kind == "lde" else L.JOB_NTT, BB if kind == "lde" else M31, L.FORWARD,
+                          log_n, batch, a, 0, dev, inp, outp)
+            t = ctypes.c_uint64()
+            ctx.check(lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+            keep.append((k, t.value, dev, din, out))
+        for k, t, dev, din, out in keep:
+            tm = L.JobTiming()
+            ctx.check(lib.sfn_server_wait(srv, t, ctypes.byref(tm)))
+            got = out.download() if dev else out
+            assert sha(got) == digests[f"serving_slice_{k}"], f"serving slice job {k}"
+            if dev:
+                din.free()
+                out.free()
+    finally:
+        lib.sfn_server_destroy(srv)

@@ -348,3 +348,28 @@ def test_coset_lde_natural_order(ctx, log_n, batch, added_bits, shift, layout):
     exp = cf.rows_bitrev(exp, log_n + added_bits, batch, row_major)
     got = api.CosetLdeBatch(ctx=ctx).execute_natural(x, log_n, batch, added_bits, shift_raw, layout).data
     assert_same(got, exp, f"natural LDE {log_n}x{batch} a={added_bits}")
+
+
+@pytest.mark.parametrize("layout", [sfntt.LAYOUT_SOA, sfntt.LAYOUT_ROW_MAJOR], ids=["soa", "rowmajor"])
+def test_chunked_host_pointer_calls(ctx, layout):
+    # host-pointer batch calls of more than 32 MiB run as column chunks on
+    # several streams (uploads, kernels and downloads overlap); row-major
+    # chunks are strided column windows moved by 2D copies
+    rm = layout == sfntt.LAYOUT_ROW_MAJOR
+    for log_n, batch, a in [(16, 300, 1), (14, 1000, 2)]:
+        x = orc.lcg_fill(6060 + log_n, batch << log_n, BB)
+        xin = x.reshape(batch, 1 << log_n).T.copy().ravel() if rm else x
+        shift = orc.bb_to_monty(31) if rm else 0
+        print(f"[parity] chunked host LDE 2^{log_n} x {batch} a={a} layout={layout}", flush=True)
+        y, ns = ctx.coset_lde(xin, log_n, batch, a, shift, layout)
+        got = y.reshape(1 << (log_n + a), batch).T.ravel() if rm else y
+        assert_same(got, orc.coset_lde_batch(x, log_n, batch, a, shift), f"chunked LDE 2^{log_n}x{batch}")
+        assert ns > 0
+    for field in (M31, KB):
+        x = data(field, 1400 << 14, 77 + field)
+        xin = x.reshape(1400, 1 << 14).T.copy().ravel() if rm else x
+        for d in (sfntt.FORWARD, sfntt.INVERSE):
+            print(f"[parity] chunked host NTT field={field} 2^14 x 1400 dir={d} layout={layout}", flush=True)
+            y, _ = ctx.ntt(field, sfntt.VARIANT_AUTO, d, 14, 1400, xin, layout)
+            got = y.reshape(1 << 14, 1400).T.ravel() if rm else y
+            assert_same(got, orc.ntt_batch(field, d == 1, 14, 1400, x), f"chunked NTT field={field} dir={d}")
