@@ -168,6 +168,25 @@ sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t b
                                         uint32_t added_bits, uint32_t shift_raw, sfn_layout layout,
                                         const uint32_t *d_in, uint32_t *d_out, void *stream);
 
+/* Allocation-free variant of sfn_coset_lde_natural_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree: the intermediate array of the
+ * row-major natural-order LDE is a static device array of 64 MiB that the
+ * library holds on every GPU it runs on until the process ends, so outputs
+ * above 64 MiB (batch << (log_n + added_bits) u32) are refused with
+ * SFN_ERR_INVALID_SIZE, and every table the call needs must already be
+ * resident: run sfn_coset_lde_natural_device once for the shape first, else
+ * SFN_ERR_NOT_READY (also after the table cache evicted them).  Calls that
+ * overlap on one GPU (other streams, other contexts, other threads) share that
+ * array and race; nothing in it needs clearing between calls (every word is
+ * written before it is read).  Same arguments and results as
+ * sfn_coset_lde_natural_device. */
+sfn_status sfn_coset_lde_natural_device_noalloc(sfn_ctx *ctx, uint32_t log_n, uint32_t batch,
+                                                uint32_t added_bits, uint32_t shift_raw,
+                                                sfn_layout layout, const uint32_t *d_in,
+                                                uint32_t *d_out, void *stream);
+
 /* Device time of the last launch sequence issued by this context on its own
  * stream (hipEvent pair), in nanoseconds; waits for it to finish.  Host-pointer
  * calls are always timed; *_device calls only while sfn_call_timing is on

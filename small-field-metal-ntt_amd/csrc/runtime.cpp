@@ -68,7 +68,35 @@ Context::~Context() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+thread_local const NoAllocScope *t_noalloc = nullptr;
+}
+
+NoAllocScope::NoAllocScope(uint32_t *s, size_t b, bool d) : scratch(s), bytes(b), dry(d), prev_(t_noalloc) {
+  t_noalloc = this;
+}
+NoAllocScope::~NoAllocScope() { t_noalloc = prev_; }
+const NoAllocScope *NoAllocScope::current() { return t_noalloc; }
+
+// a dry run plans the call and looks every table up, but launches nothing
+static bool skip_launch() {
+  const NoAllocScope *na = NoAllocScope::current();
+  return na && na->dry;
+}
+
+static sfn_status not_resident(Context *c, const char *what) {
+  return c->fail(SFN_ERR_NOT_READY, std::string(what) +
+                                        " is not resident and this entry point allocates nothing: run the allocating "
+                                        "entry point once for this shape first");
+}
+
 sfn_status Context::acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **out) {
+  if (const NoAllocScope *na = NoAllocScope::current()) {
+    // the library's static scratch: calls that overlap on one device race on it
+    if (bytes > na->bytes) return fail(SFN_ERR_INVALID_SIZE, "LDE intermediate exceeds the static scratch");
+    *out = na->scratch;
+    return SFN_OK;
+  }
   if (!scratch_ev && hipEventCreateWithFlags(&scratch_ev, hipEventDisableTiming) != hipSuccess) {
     scratch_ev = nullptr;
     return fail(SFN_ERR_GPU_EXECUTION, "scratch event");
@@ -82,11 +110,13 @@ sfn_status Context::acquire_lde_scratch(size_t bytes, hipStream_t s, uint32_t **
 }
 
 void Context::release_lde_scratch(hipStream_t s) {
+  if (NoAllocScope::current()) return;
   if (hipEventRecord(scratch_ev, s) == hipSuccess) scratch_ev_used = true;
 }
 
 sfn_status Context::ensure(DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes && b.ptr) return SFN_OK;
+  if (NoAllocScope::current()) return not_resident(this, "a scratch buffer of this size");
   if (b.ptr) {
     // kernels on any stream may still read the old buffer (tables are shared
     // by every stream of the context); growth is rare, so wait for the device
@@ -181,6 +211,7 @@ void Context::evict_for(size_t incoming) {
 sfn_status Context::cached_table(const TableKey &k, TableBuilder build, const void **p0, const void **p1) {
   std::lock_guard<std::recursive_mutex> g(mu);
   auto it = tables.find(k);
+  if (it == tables.end() && NoAllocScope::current()) return not_resident(this, "a twiddle table of this shape");
   if (it == tables.end()) {
     std::vector<uint32_t> h0, h1;
     build(k, h0, h1);
@@ -225,7 +256,7 @@ void pair_words(const std::vector<host::Pair> &v, std::vector<uint32_t> &out) {
 }  // namespace
 
 hipError_t Context::timing_mark(hipStream_t s, bool before, const std::string &label) {
-  if (!pass_timing || s != stream) return hipSuccess;
+  if (!pass_timing || s != stream || NoAllocScope::current()) return hipSuccess;
   const size_t need = 2 * (size_t)(tcount + 1);
   while (tev.size() < need) {
     hipEvent_t e;
@@ -248,6 +279,7 @@ sfn_status Context::position_table(int field, uint32_t log_max, const uint2 **fw
   std::lock_guard<std::recursive_mutex> g(mu);
   const int fi = mf_index(field);
   if (log_max < kMaxLogTile) log_max = kMaxLogTile;
+  if (mf_pos_log[fi] < log_max && NoAllocScope::current()) return not_resident(this, "a position table");
   if (mf_pos_log[fi] < log_max) {
     for (int d = 0; d < 2; ++d) {
       auto t = host::position_table(host::monty_field(field), log_max, d == 1);
@@ -269,6 +301,7 @@ sfn_status Context::position_table(int field, uint32_t log_max, const uint2 **fw
 
 sfn_status Context::bb_small_pair_table(const uint2 **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
+  if (!bb_small_pair.ptr && NoAllocScope::current()) return not_resident(this, "the small twiddle table");
   if (!bb_small_pair.ptr) {
     auto f = host::position_table(host::kBabyBear, kMaxLogTile, false);
     auto i = host::position_table(host::kBabyBear, kMaxLogTile, true);
@@ -284,6 +317,7 @@ sfn_status Context::bb_small_pair_table(const uint2 **out) {
 sfn_status Context::root_table(int field, const uint2 **out) {
   std::lock_guard<std::recursive_mutex> g(mu);
   DevBuf &r = mf_root[mf_index(field)];
+  if (!r.ptr && NoAllocScope::current()) return not_resident(this, "the root table");
   if (!r.ptr) {
     auto t = host::root_table(host::monty_field(field), kMaxLogTile);
     sfn_status s = upload_table(r, t.data(), t.size() * sizeof(host::Pair));
@@ -775,7 +809,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
       scale_factor(field, 1ull << log_n_real, &a.scale_w, &a.scale_q);
     }
     (void)ctx->timing_mark(s, true, "k_layer");
-    hipError_t e = launch_layer(field, dir, a, s);
+    hipError_t e = skip_launch() ? hipSuccess : launch_layer(field, dir, a, s);
     if (e != hipSuccess) return ctx->hip_fail(e, "layer kernel launch");
     (void)ctx->timing_mark(s, false, "");
     return SFN_OK;
@@ -793,7 +827,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     (void)ctx->timing_mark(s, true, std::string("k_fast<") + field_tag(field) + (dir ? "inv," : "fwd,") +
                                        std::to_string(p.log_t) + (p.lane_contig ? ",lanes" : ",points") +
                                        ">lo=" + std::to_string(p.lo));
-    hipError_t e = launch_fast(field, dir, p.log_t, p.lane_contig != 0, a, p.tiles, s);
+    hipError_t e = skip_launch() ? hipSuccess : launch_fast(field, dir, p.log_t, p.lane_contig != 0, a, p.tiles, s);
     if (e != hipSuccess) return ctx->hip_fail(e, "tuned pass launch");
     (void)ctx->timing_mark(s, false, "");
     return SFN_OK;
@@ -809,7 +843,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
   (void)ctx->timing_mark(s, true, std::string("k_pass<") + field_tag(field) + (dir ? "inv," : "fwd,") +
                                      std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
                                      std::to_string(p.lo));
-  hipError_t e = launch_pass(field, dir, p.log_t, p.logr, a, p.tiles, s);
+  hipError_t e = skip_launch() ? hipSuccess : launch_pass(field, dir, p.log_t, p.logr, a, p.tiles, s);
   if (e != hipSuccess) return ctx->hip_fail(e, "pass kernel launch");
   (void)ctx->timing_mark(s, false, "");
   return SFN_OK;
@@ -821,7 +855,7 @@ sfn_status Context::run_ntt(int field, int variant, int dir, uint32_t log_n, uin
   const uint64_t total = (uint64_t)batch << log_n;
   if (log_n == 0) {  // identity (src/ntt/metal_r2.rs:81-83)
     if (d_in != d_out) {
-      hipError_t e = launch_copy(d_in, d_out, total, s);
+      hipError_t e = skip_launch() ? hipSuccess : launch_copy(d_in, d_out, total, s);
       if (e != hipSuccess) return hip_fail(e, "copy launch");
     }
     return SFN_OK;
@@ -871,7 +905,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     // copy with a zero source row stride broadcasts the input row.
     const Layout Lo = layout_of(layout, LN, batch);
     (void)timing_mark(s, true, "k_rows_bitrev<broadcast>");
-    hipError_t e = launch_rows_bitrev(d_in, d_out, LN, batch, 1, 0, Lo.cs, Lo.rs, s);
+    hipError_t e = skip_launch() ? hipSuccess : launch_rows_bitrev(d_in, d_out, LN, batch, 1, 0, Lo.cs, Lo.rs, s);
     if (e != hipSuccess) return hip_fail(e, "LDE broadcast launch");
     (void)timing_mark(s, false, "");
     return SFN_OK;
@@ -919,7 +953,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     // natural-order input: its rows, bit-reversed, become the first n rows of
     // out, where the transform then runs in place
     (void)timing_mark(s, true, "k_rows_bitrev<in>");
-    hipError_t e = launch_rows_bitrev(d_in, d_out, log_n, batch, Lin.cs, Lin.rs, Lout.cs, Lout.rs, s);
+    hipError_t e = skip_launch() ? hipSuccess : launch_rows_bitrev(d_in, d_out, log_n, batch, Lin.cs, Lin.rs, Lout.cs, Lout.rs, s);
     if (e != hipSuccess) return hip_fail(e, "row permutation launch");
     (void)timing_mark(s, false, "");
     mid_src = d_out;
@@ -1003,7 +1037,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       a.lane_contig = 1;
       a.xcd_share_log = xcd_share_log(this, q);
       (void)timing_mark(s, true, "k_mid_fast<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
-      hipError_t e = launch_lde_mid_fast((int)mid_t, a, q.tiles, s);
+      hipError_t e = skip_launch() ? hipSuccess : launch_lde_mid_fast((int)mid_t, a, q.tiles, s);
       if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
       (void)timing_mark(s, false, "");
     } else {
@@ -1011,7 +1045,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       a.tw = pair;
       const uint64_t tiles = (a.instances + (1ull << lw) - 1) >> lw;
       (void)timing_mark(s, true, "k_lde_mid<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
-      hipError_t e = launch_lde_mid((int)mid_t, a, tiles, s);
+      hipError_t e = skip_launch() ? hipSuccess : launch_lde_mid((int)mid_t, a, tiles, s);
       if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
       (void)timing_mark(s, false, "");
     }
@@ -1032,7 +1066,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
   if (fused) release_lde_scratch(s);
   if (natural && !fused) {  // bit-reversed output rows -> natural order, in place
     (void)timing_mark(s, true, "k_rows_bitrev<out>");
-    hipError_t e = launch_rows_bitrev(d_out, d_out, LN, batch, Lout.cs, Lout.rs, Lout.cs, Lout.rs, s);
+    hipError_t e = skip_launch() ? hipSuccess : launch_rows_bitrev(d_out, d_out, LN, batch, Lout.cs, Lout.rs, Lout.cs, Lout.rs, s);
     if (e != hipSuccess) return hip_fail(e, "row permutation launch");
     (void)timing_mark(s, false, "");
   }
@@ -1181,6 +1215,35 @@ hipError_t launch_multi_any(const MultiLaunch &L, const SegDesc *d_segs, uint32_
 
 // ======================================================================= C ABI
 using sfn::Context;
+
+namespace {
+// Intermediate array of sfn_coset_lde_natural_device_noalloc: a static device
+// array, so that entry point allocates nothing.  Every GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns for that device (kept per device).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchBytes / 4];
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: the address of this device's copy of `scratch`.
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidValue;
+  g_scratch_ptr[device] = (uint32_t *)p;
+  return hipSuccess;
+}
+
+uint32_t *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
 
 static uint64_t now_ns() {
   return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -1370,6 +1433,11 @@ sfn_status sfn_ctx_create(int device, sfn_ctx **out) {
   if (strncmp(x.props.gcnArchName, "gfx950", 6) != 0) {
     delete c;
     return SFN_ERR_DEVICE_NOT_FOUND;  // the kernels are built for gfx950 only
+  }
+  if (scratch_setup(device) != hipSuccess) {  // this device's static scratch (allocation-free entry point)
+    (void)hipGetLastError();
+    delete c;
+    return SFN_ERR_GPU_EXECUTION;
   }
   *out = c;
   return SFN_OK;
@@ -1607,6 +1675,38 @@ sfn_status sfn_coset_lde_natural_device(sfn_ctx *ctx, uint32_t log_n, uint32_t b
   if (sfn::ranges_overlap(d_in, ((size_t)batch << log_n) * 4, d_out, ((size_t)batch << (log_n + added_bits)) * 4))
     return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
   (void)hipSetDevice(c.device);
+  hipStream_t s = pick_stream(c, stream);
+  const bool own = s == c.stream, timed = own && c.call_timing;
+  if (timed) (void)hipEventRecord(c.ev0, s);
+  st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, s, nullptr, true);
+  if (timed) (void)hipEventRecord(c.ev1, s);
+  if (own) c.timing_pending = timed && st == SFN_OK;
+  return st;
+}
+
+sfn_status sfn_coset_lde_natural_device_noalloc(sfn_ctx *ctx, uint32_t log_n, uint32_t batch, uint32_t added_bits,
+                                                uint32_t shift_raw, sfn_layout layout, const uint32_t *d_in,
+                                                uint32_t *d_out, void *stream) {
+  if (!ctx) return SFN_ERR_INVALID_ARGUMENT;
+  Context &c = ctx->impl;
+  sfn_status st = sfn::validate_lde(&c, log_n, batch, added_bits, layout);
+  if (st) return st;
+  if (!d_in || !d_out) return c.fail(SFN_ERR_INVALID_ARGUMENT, "null buffer");
+  const size_t in_bytes = ((size_t)batch << log_n) * 4, out_bytes = ((size_t)batch << (log_n + added_bits)) * 4;
+  if (sfn::ranges_overlap(d_in, in_bytes, d_out, out_bytes))
+    return c.fail(SFN_ERR_INVALID_ARGUMENT, "LDE input and output must not overlap");
+  if (out_bytes > kNoAllocScratchBytes)
+    return c.fail(SFN_ERR_INVALID_SIZE, "output of " + std::to_string(out_bytes) +
+                                            " bytes exceeds the allocation-free LDE's static scratch of " +
+                                            std::to_string(kNoAllocScratchBytes) + " bytes");
+  uint32_t *scr = scratch_of(c.device);
+  if (!scr) return c.fail(SFN_ERR_NOT_READY, "static scratch address not set up for this device");
+  (void)hipSetDevice(c.device);
+  {  // every table resident?  A dry run looks them up without launching anything
+    sfn::NoAllocScope dry(scr, kNoAllocScratchBytes, true);
+    if ((st = c.run_lde(log_n, batch, added_bits, shift_raw, layout, d_in, d_out, c.stream, nullptr, true))) return st;
+  }
+  sfn::NoAllocScope na(scr, kNoAllocScratchBytes, false);
   hipStream_t s = pick_stream(c, stream);
   const bool own = s == c.stream, timed = own && c.call_timing;
   if (timed) (void)hipEventRecord(c.ev0, s);

@@ -82,6 +82,27 @@ struct ChunkStreams {
   }
 };
 
+// While a NoAllocScope is open on a thread (the allocation-free entry point),
+// the runtime allocates no device memory: a table or scratch buffer that is
+// not resident fails the call with SFN_ERR_NOT_READY instead of being built,
+// and the natural-order LDE's intermediate array is `scratch` (the library's
+// static device array, `bytes` long) instead of the context's.  A dry scope
+// also launches nothing (the call only finds out whether it could run).
+class NoAllocScope {
+ public:
+  NoAllocScope(uint32_t *scratch, size_t bytes, bool dry);
+  ~NoAllocScope();
+  NoAllocScope(const NoAllocScope &) = delete;
+  NoAllocScope &operator=(const NoAllocScope &) = delete;
+  static const NoAllocScope *current();
+  uint32_t *scratch;
+  size_t bytes;
+  bool dry;
+
+ private:
+  const NoAllocScope *prev_;
+};
+
 class Context;
 // While a TablePins scope is open on a thread, every cached table the thread
 // obtains from `ctx` stays resident until the scope closes (its launches are

@@ -88,6 +88,9 @@ SIGNATURES = {
     "sfn_coset_lde_natural_device": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                                   vp, vp, vp]),
+    "sfn_coset_lde_natural_device_noalloc": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32,
+                                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                                         vp, vp, vp]),
     "sfn_last_device_ns": (ctypes.c_int, [vp, u64p]),
     "sfn_call_timing": (ctypes.c_int, [vp, ctypes.c_int]),
     "sfn_pass_timing": (ctypes.c_int, [vp, ctypes.c_int]),

@@ -211,8 +211,9 @@ hipError_t launch_kernel(void (*k)(P...), dim3 grid, dim3 block, size_t lds, hip
 extern "C" {
 // buffer-resource accesses past the resource's extent since the last reset
 uint64_t sfn_model_oob_count();
-// kernel launches since the last reset
+// kernel launches since the last reset; hipMalloc / hipHostMalloc calls ever
 uint64_t sfn_model_launch_count();
+uint64_t sfn_model_malloc_count();
 // bytes held by live hipMalloc allocations, and their peak since the last reset
 uint64_t sfn_model_device_bytes();
 uint64_t sfn_model_peak_device_bytes();

@@ -44,7 +44,7 @@ alignas(64) unsigned char smem[kLdsBytes];
 
 namespace {
 
-std::atomic<uint64_t> g_oob{0}, g_launches{0};
+std::atomic<uint64_t> g_oob{0}, g_launches{0}, g_mallocs{0};
 std::mutex g_launch_mu;  // one kernel at a time
 thread_local hipError_t t_last = hipSuccess;
 
@@ -340,6 +340,7 @@ hipError_t hipDeviceSynchronize() { return hipSuccess; }
 
 hipError_t hipMalloc(void **p, size_t bytes) {
   *p = nullptr;
+  g_mallocs.fetch_add(1);
   if (inject("malloc")) {
     set_err(hipErrorOutOfMemory);
     return hipErrorOutOfMemory;
@@ -372,6 +373,7 @@ hipError_t hipFree(void *p) {
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned flags) {
   (void)flags;
   *p = nullptr;
+  g_mallocs.fetch_add(1);
   if (inject("host_malloc")) return hipErrorOutOfMemory;
   if (posix_memalign(p, 256, bytes ? bytes : 1) != 0) return hipErrorOutOfMemory;
   return hipSuccess;
@@ -515,6 +517,7 @@ hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
 extern "C" {
 uint64_t sfn_model_oob_count() { return g_oob.load(); }
 uint64_t sfn_model_launch_count() { return g_launches.load(); }
+uint64_t sfn_model_malloc_count() { return g_mallocs.load(); }
 uint64_t sfn_model_device_bytes() {
   std::lock_guard<std::mutex> lk(g_mem_mu);
   return g_dev_bytes;

@@ -43,6 +43,7 @@ lib.sfn_model_oob_count.restype = ctypes.c_uint64
 lib.sfn_model_launch_count.restype = ctypes.c_uint64
 lib.sfn_model_device_bytes.restype = ctypes.c_uint64
 lib.sfn_model_peak_device_bytes.restype = ctypes.c_uint64
+lib.sfn_model_malloc_count.restype = ctypes.c_uint64
 lib.sfn_model_set_faults.argtypes = [ctypes.c_char_p]
 
 M31, BB, KB = L.FIELD_M31, L.FIELD_BABYBEAR, L.FIELD_KOALABEAR
@@ -459,10 +460,52 @@ def suite_faults(ctx):
     check(lib.sfn_model_device_bytes() == 0, f"{lib.sfn_model_device_bytes()} device bytes still held")
 
 
+def suite_noalloc(ctx):
+    """The allocation-free natural-order LDE makes no device allocation once the
+    shape's tables are resident, gives the allocating entry point's bits, and
+    refuses shapes whose tables are missing or whose output exceeds its scratch."""
+    import closed_forms as cf
+    c = sfntt.HipContext(0)
+    fn, ref = lib.sfn_coset_lde_natural_device_noalloc, lib.sfn_coset_lde_natural_device
+    for log_n, batch, a, layout in ((13, 16, 1, L.LAYOUT_ROW_MAJOR), (10, 8, 2, L.LAYOUT_ROW_MAJOR),
+                                    (12, 4, 1, L.LAYOUT_SOA)):
+        rm = layout == L.LAYOUT_ROW_MAJOR
+        shift = orc.bb_to_monty(31)
+        x = orc.lcg_fill(808 + log_n, batch << log_n, BB)
+        nout = batch << (log_n + a)
+        din, dref, dna = sfntt.DeviceBuffer(c, x.size), sfntt.DeviceBuffer(c, nout), sfntt.DeviceBuffer(c, nout)
+        din.upload(x)
+        log(f"[noalloc] 2^{log_n}x{batch} a={a} layout={layout}")
+        c.check(ref(c.ptr, log_n, batch, a, shift, layout, din.ptr, dref.ptr, None))
+        before = lib.sfn_model_malloc_count()
+        c.check(fn(c.ptr, log_n, batch, a, shift, layout, din.ptr, dna.ptr, None))
+        check(lib.sfn_model_malloc_count() == before, "the allocation-free call allocated")
+        exp = orc.coset_lde_batch(cf.rows_bitrev(x, log_n, batch, rm), log_n, batch, a, shift, layout=layout)
+        exp = cf.rows_bitrev(exp, log_n + a, batch, rm)
+        same(dna.download(), dref.download(), f"noalloc vs allocating 2^{log_n}x{batch}")
+        same(dna.download(), exp, f"noalloc vs oracle 2^{log_n}x{batch}")
+        for b in (din, dref, dna):
+            b.free()
+    d = sfntt.DeviceBuffer(c, 3 << 20)
+    launches = lib.sfn_model_launch_count()
+    rc = fn(c.ptr, 11, 8, 1, orc.bb_to_monty(31), L.LAYOUT_ROW_MAJOR, d.ptr,
+            ctypes.c_void_p(d.ptr.value + (8 << 13)), None)
+    check(rc == L.ERR_NOT_READY, f"missing tables: status {rc}")
+    check(lib.sfn_model_launch_count() == launches, "a refused call launched kernels")
+    rc = fn(c.ptr, 20, 16, 1, 0, L.LAYOUT_ROW_MAJOR, d.ptr, ctypes.c_void_p(d.ptr.value + 4), None)
+    check(rc == L.ERR_INVALID_ARGUMENT, f"overlap: status {rc}")
+    big = sfntt.DeviceBuffer(c, 48 << 20)
+    rc = fn(c.ptr, 20, 16, 1, 0, L.LAYOUT_ROW_MAJOR, big.ptr, ctypes.c_void_p(big.ptr.value + (64 << 20)), None)
+    check(rc == L.ERR_INVALID_SIZE, f"over the scratch: status {rc}")
+    d.free()
+    big.free()
+    c.close()
+
+
 SUITES = {"ntt": suite_ntt, "lde": suite_lde, "schedules": suite_schedules,
           "schedules_full": lambda ctx: suite_schedules(ctx, full=True), "plans": suite_plans,
           "server": lambda ctx: (suite_server(ctx), suite_server_failures(ctx)), "faults": suite_faults,
-          "cache": suite_cache}
+          "cache": suite_cache, "noalloc": suite_noalloc}
 
 
 def main(argv):

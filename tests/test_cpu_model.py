@@ -35,7 +35,7 @@ def model_env():
     return env
 
 
-SUITES = ["ntt", "lde", "plans", "server", "faults", "cache",
+SUITES = ["ntt", "lde", "plans", "server", "faults", "cache", "noalloc",
           "schedules_full" if os.environ.get("SFNTT_MODEL_FULL") == "1" else "schedules"]
 
 
