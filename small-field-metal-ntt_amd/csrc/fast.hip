@@ -358,7 +358,10 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
 // the twist, point-factor and post tables it needs are T words each; they are
 // staged in LDS ([tables: 2T words] between the root table and the tile)
 // instead of being read per lane from global memory.
-template <int LOG_T, bool LT, int LW = -1>
+// CONTIG: lanes adjacent in memory (the middle layers of a multi-pass LDE);
+// otherwise every lane is one whole column of points adjacent in memory (an
+// LDE of at most 2^11 rows, SoA: lo == 0, 16-byte vector loads and stores).
+template <int LOG_T, bool LT, int LW = -1, bool CONTIG = true>
 __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   using F = BabyBear;
   using S = Sched<LOG_T, KR>;
@@ -390,16 +393,17 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   RootTw<LOG_T, 0> twf;
   twi.tab = twf.tab = reinterpret_cast<const uint2 *>(tabp);
   // inverse rounds 0 .. NR-2 (round 0 loads from HBM with the inverse twist)
-  rounds<F, 1, LOG_T, true, 0, S::NR - 1, LT>(tile, log_w, lanes, twi, io);
+  rounds<F, 1, LOG_T, CONTIG, 0, S::NR - 1, LT>(tile, log_w, lanes, twi, io);
   __syncthreads();
   // inverse last round: one item per thread (blockDim == W << (LOG_T - K))
   const uint32_t W = 1u << log_w;
   const uint32_t it = threadIdx.x;
-  const uint32_t lane = it & (W - 1), rb = it >> log_w;
+  const uint32_t lane = CONTIG ? it & (W - 1) : it >> (LOG_T - K);
+  const uint32_t rb = CONTIG ? it >> log_w : it & ((1u << (LOG_T - K)) - 1);
   const uint32_t pbase = round_pbase<K, B0>(rb);
   uint32_t c[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) c[i] = tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)];
+  for (int i = 0; i < R; ++i) c[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
   round_regs<F, 1, K, B0>(c, pbase, twi);
   const uint64_t n = 1ull << a.log_n;
   io.has_twist = false;  // the forward twist is part of post_b
@@ -432,7 +436,7 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
     round_regs<F, 0, K, B0>(v, pbase, twf);
     __syncthreads();  // every thread is done reading the tile (and the tables in use)
 #pragma unroll
-    for (int i = 0; i < R; ++i) tile[lidx<LOG_T, true>(lane, pbase | (i << B0), log_w)] = v[i];
+    for (int i = 0; i < R; ++i) tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)] = v[i];
     if constexpr (LT) {
       if (pre) {
         if (e4 < T) {
@@ -449,7 +453,8 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
     }
     __syncthreads();
     io.dst = make_rsrc(a.dst + b * a.dst_block_off + io.dbase0, (a.dst_elems - b * a.dst_block_off - io.dbase0) * 4);
-    rounds<F, 0, LOG_T, true, 1, S::NR, LT>(tile, log_w, lanes, twf, io);
+    io.dstp = a.dst + b * a.dst_block_off + io.dbase0;
+    rounds<F, 0, LOG_T, CONTIG, 1, S::NR, LT>(tile, log_w, lanes, twf, io);
   }
 }
 
@@ -531,48 +536,51 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
   return dir == 0 ? launch_fast_bb_fwd(log_t, contig, a, tiles, s) : launch_fast_bb_inv(log_t, contig, a, tiles, s);
 }
 
-template <int LOG_T, bool LT>
+template <int LOG_T, bool LT, bool CONTIG>
 static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t lds, hipStream_t s) {
   // one-time attribute set-up; a function-local static initialises once,
   // also when several host threads launch at the same time
   static const bool configured = [&] {
-    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxLdsBytes);
-    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 3>,
+    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, -1, CONTIG>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 3, CONTIG>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     return true;
   }();
   (void)configured;
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
-  // eight lanes (the default middle-pass width): lane count fixed at compile time
-  if (LOG_T == 10 && a.log_w == 3)
-    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, 3>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+  // eight adjacent lanes (the default middle-pass width): lane count fixed at compile time
+  if (CONTIG && LOG_T == 10 && a.log_w == 3)
+    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, 3, CONTIG>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds,
+                       s, a);
   else
-    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
+    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, -1, CONTIG>), dim3((uint32_t)tiles), dim3((uint32_t)threads),
+                       lds, s, a);
   return hipGetLastError();
 }
 
 // LDS-staged tables when the lanes share their low row bits (tab_low_major)
 // and the extra 2T words fit; otherwise per-lane global table reads.
 template <int LOG_T>
-static hipError_t launch_mid_fast_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
+static hipError_t launch_mid_fast_t(bool contig, const PassArgs &a, uint64_t tiles, hipStream_t s) {
   const size_t lds = fast::lds_bytes(1, LOG_T, a.log_w);
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
   if (lds > kMaxLdsBytes || threads > 512 || tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
+  if (!contig) return launch_mid_fast_lt<LOG_T, false, false>(a, tiles, lds, s);
   const size_t lds_lt = lds + ((size_t)8 << LOG_T);
-  if (a.tab_low_major && lds_lt <= kMaxLdsBytes) return launch_mid_fast_lt<LOG_T, true>(a, tiles, lds_lt, s);
-  return launch_mid_fast_lt<LOG_T, false>(a, tiles, lds, s);
+  if (a.tab_low_major && lds_lt <= kMaxLdsBytes) return launch_mid_fast_lt<LOG_T, true, true>(a, tiles, lds_lt, s);
+  return launch_mid_fast_lt<LOG_T, false, true>(a, tiles, lds, s);
 }
 
-hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s) {
+hipError_t launch_lde_mid_fast(int log_t, bool contig, const PassArgs &a, uint64_t tiles, hipStream_t s) {
   if (tiles == 0) return hipSuccess;
   switch (log_t) {
-    case 6: return launch_mid_fast_t<6>(a, tiles, s);
-    case 7: return launch_mid_fast_t<7>(a, tiles, s);
-    case 8: return launch_mid_fast_t<8>(a, tiles, s);
-    case 9: return launch_mid_fast_t<9>(a, tiles, s);
-    case 10: return launch_mid_fast_t<10>(a, tiles, s);
-    case 11: return launch_mid_fast_t<11>(a, tiles, s);
+    case 6: return launch_mid_fast_t<6>(contig, a, tiles, s);
+    case 7: return launch_mid_fast_t<7>(contig, a, tiles, s);
+    case 8: return launch_mid_fast_t<8>(contig, a, tiles, s);
+    case 9: return launch_mid_fast_t<9>(contig, a, tiles, s);
+    case 10: return launch_mid_fast_t<10>(contig, a, tiles, s);
+    case 11: return launch_mid_fast_t<11>(contig, a, tiles, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -99,7 +99,9 @@ hipError_t launch_multi(const SegDesc *d_segs, uint32_t nseg, uint64_t total_til
 size_t fast_lds_bytes(int field, int log_t, uint32_t log_w);
 hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArgs &a, uint64_t tiles,
                        hipStream_t s);
-hipError_t launch_lde_mid_fast(int log_t, const PassArgs &a, uint64_t tiles, hipStream_t s);
+// contig: lanes adjacent in memory; otherwise each lane is one column of
+// adjacent points (lo == 0, 16-byte aligned columns)
+hipError_t launch_lde_mid_fast(int log_t, bool contig, const PassArgs &a, uint64_t tiles, hipStream_t s);
 // Tuned multi-job pass: segments' kind = field * 2 + dir (+ 4 when lanes are
 // adjacent in memory), log_t in [6, 11]; kMultiFastThreads threads per
 // workgroup.  bb_root: the BabyBear root table (2^11 Shoup pairs) when any

@@ -1017,8 +1017,12 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     q.log_t = (int)mid_t;
     q.log_n = log_n;
     Layout mid_ld = Lout;  // per block the kernel rebases at the block start
-    if (allow_fast && sc.logr == 5 && choose_lanes_fast(q, layout, batch, sc.mid_log_w) && q.lane_contig &&
-        fast_offsets_fit(q, mid_ls, mid_ld)) {
+    // lanes adjacent in memory; or, when the middle pass is the whole transform
+    // (lo == 0, SoA), one column per lane with 16-byte vector accesses.  With
+    // outer passes the tuned middle pass goes with their radix-32 schedule.
+    const bool mid_aligned = ((uintptr_t)mid_src % 16 == 0) && ((uintptr_t)work % 16 == 0);
+    if (allow_fast && (sc.logr == 5 || lo == 0) && choose_lanes_fast(q, layout, batch, sc.mid_log_w) &&
+        (q.lane_contig || (lo == 0 && mid_aligned)) && fast_offsets_fit(q, mid_ls, mid_ld)) {
       const uint2 *root;
       const uint32_t *tim, *post, *pointf;
       if ((st = root_table(SFN_FIELD_BABYBEAR, &root))) return st;
@@ -1027,17 +1031,20 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, low_major, &post, &pointf))) return st;
       a.tab_low_major = low_major;
       a.tw = root;
-      a.twist_m = tim;
+      // one column per lane covers the whole transform: no four-step twist
+      a.twist_m = q.lane_contig ? tim : nullptr;
       a.post_m = post;
       a.pointf_m = pointf;
       a.src_elems = mid_src == d_in ? n_in : n_out;
       a.dst_elems = n_out;
       a.log_w = q.log_w;
       a.order = q.order;
-      a.lane_contig = 1;
+      a.lane_contig = q.lane_contig;
       a.xcd_share_log = xcd_share_log(this, q);
-      (void)timing_mark(s, true, "k_mid_fast<" + std::to_string(mid_t) + ">lo=" + std::to_string(lo));
-      hipError_t e = skip_launch() ? hipSuccess : launch_lde_mid_fast((int)mid_t, a, q.tiles, s);
+      (void)timing_mark(s, true, "k_mid_fast<" + std::to_string(mid_t) + (q.lane_contig ? ",lanes" : ",points") +
+                                     ">lo=" + std::to_string(lo));
+      hipError_t e =
+          skip_launch() ? hipSuccess : launch_lde_mid_fast((int)mid_t, q.lane_contig != 0, a, q.tiles, s);
       if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
       (void)timing_mark(s, false, "");
     } else {

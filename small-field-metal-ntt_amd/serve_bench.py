@@ -53,20 +53,26 @@ def draw_jobs(seed: int, count: int, scale: int = 0):
 LOADGEN = os.path.join(HERE, "sfntt", "serve_load")
 
 
-def run_load(jobs, arrival_sets, system: str, mode: str, inflight: int, group: int):
-    """Replays every arrival sequence through the native load generator
-    (serve_load.cpp): one process, the shapes warmed once; returns
-    [(latencies in s, span in s, mean group)] per sequence."""
+def write_trace(path, jobs, arrival_sets):
+    """The load generator's trace file (format in serve_load.cpp)."""
     import struct
-    import subprocess
-    import tempfile
-    with tempfile.NamedTemporaryFile(suffix=".trace", delete=False) as fh:
+    with open(path, "wb") as fh:
         fh.write(struct.pack("<II", len(jobs), len(arrival_sets)))
         for kind, n, b, a in jobs:
             fh.write(struct.pack("<6I", 1 if kind == "lde" else 0, 1 if kind == "lde" else 0, n, b, a, 0))
         for arr in arrival_sets:
             fh.write(np.asarray(arr, dtype="<f8").tobytes())
+
+
+def run_load(jobs, arrival_sets, system: str, mode: str, inflight: int, group: int):
+    """Replays every arrival sequence through the native load generator
+    (serve_load.cpp): one process, the shapes warmed once; returns
+    [(latencies in s, span in s, mean group)] per sequence."""
+    import subprocess
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".trace", delete=False) as fh:
         path = fh.name
+    write_trace(path, jobs, arrival_sets)
     try:
         r = subprocess.run([LOADGEN, path, system, mode, str(inflight), str(group)], capture_output=True, text=True,
                            check=True)
@@ -109,9 +115,15 @@ def main():
     ap.add_argument("--inflight", type=int, default=4)
     ap.add_argument("--group", type=int, default=64, help="most jobs the server fuses into one launch set")
     ap.add_argument("--modes", default="device,host")
+    ap.add_argument("--write-trace", default="",
+                    help="only write the trace (every job arriving at once, closed loop) to this file, e.g. to "
+                         "replay it under a profiler: sfntt/serve_load FILE plain|server device|host")
     args = ap.parse_args()
     modes = [m for m in args.modes.split(",") if m]
     jobs = draw_jobs(0x5EED0005, args.jobs, args.scale)
+    if args.write_trace:
+        write_trace(args.write_trace, jobs, [np.zeros(len(jobs))])
+        return
     rng = np.random.default_rng(0x5EED0005 + 1)
     gaps = rng.exponential(1.0, len(jobs))
     targets = [float(t) for t in args.p99_ms.split(",") if t]

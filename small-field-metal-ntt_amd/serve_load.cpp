@@ -3,7 +3,7 @@
 // per job, synchronous) or the server (sfn_server_*), natively, so the
 // driver's own per-job cost stays far below the jobs' cost.
 //
-//   serve_load <trace> <plain|server> <device|host> [inflight] [max_group]
+//   serve_load <trace> <plain|server> <device|host> [inflight] [max_group] [busy]
 //
 // <trace>: little-endian u32 job count J and u32 rate count R, then J records
 // of 6 u32 (kind 0 = NTT / 1 = LDE, field, log_n, batch, added_bits, 0), then
@@ -15,6 +15,9 @@
 // staging).  Prints one JSON line per arrival sequence: per-job latency
 // (arrival -> result in place) in microseconds, the span from the first
 // arrival to the last completion, and the mean fused group size (server).
+// With "busy" the plain path also times every call's launch sequence on the
+// device (hipEvent pair, sfn_call_timing) and reports their sum over the span
+// as device_busy: the share of the time the GPU had work.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -63,6 +66,7 @@ int main(int argc, char **argv) {
   const bool device = std::string(argv[3]) == "device";
   const uint32_t inflight = argc > 4 ? (uint32_t)atoi(argv[4]) : 4;
   const uint32_t max_group = argc > 5 ? (uint32_t)atoi(argv[5]) : 0;
+  const bool busy = argc > 6 && std::string(argv[6]) == "busy";
   std::vector<Job> jobs;
   std::vector<std::vector<double>> arrivals;
   {
@@ -105,8 +109,10 @@ int main(int argc, char **argv) {
     memcpy(h_in, host_in.data(), max_in * 4);
   }
   CHECK(sfn_synchronize(ctx, nullptr));
+  double device_ns = 0;  // "busy": the launch sequences' device time
   auto plain_call = [&](size_t k) {
     const Job &j = jobs[k];
+    uint64_t ns = 0;
     if (device) {
       if (j.kind == 1)
         CHECK(sfn_coset_lde_batch_device(ctx, j.log_n, j.batch, j.added_bits, 0, SFN_LAYOUT_SOA, (const uint32_t *)d_in,
@@ -115,14 +121,17 @@ int main(int argc, char **argv) {
         CHECK(sfn_ntt_device(ctx, (sfn_field)j.field, SFN_VARIANT_AUTO, SFN_FORWARD, j.log_n, j.batch, SFN_LAYOUT_SOA,
                              (const uint32_t *)d_in, (uint32_t *)d_out[k], nullptr));
       CHECK(sfn_synchronize(ctx, nullptr));
+      if (busy) CHECK(sfn_last_device_ns(ctx, &ns));
     } else if (j.kind == 1) {
       CHECK(sfn_coset_lde_batch(ctx, j.log_n, j.batch, j.added_bits, 0, SFN_LAYOUT_SOA, (const uint32_t *)h_in,
-                                (uint32_t *)h_out, nullptr));
+                                (uint32_t *)h_out, &ns));
     } else {
       CHECK(sfn_ntt(ctx, (sfn_field)j.field, SFN_VARIANT_AUTO, SFN_FORWARD, j.log_n, j.batch, SFN_LAYOUT_SOA,
-                    (const uint32_t *)h_in, (uint32_t *)h_out, nullptr));
+                    (const uint32_t *)h_in, (uint32_t *)h_out, &ns));
     }
+    device_ns += (double)ns;
   };
+  if (busy) CHECK(sfn_call_timing(ctx, 1));
   auto request = [&](size_t k) {
     const Job &j = jobs[k];
     sfn_request r{};
@@ -158,6 +167,7 @@ int main(int argc, char **argv) {
     double span = 0, mean_group = 1;
     const uint64_t t0 = now_ns() + 1000000;
     uint64_t last = t0;
+    device_ns = 0;
     if (!server) {
       for (size_t k = 0; k < jobs.size(); ++k) {
         const uint64_t at = t0 + (uint64_t)(arr[k] * 1e9);
@@ -187,7 +197,9 @@ int main(int argc, char **argv) {
       mean_group = groups / (double)jobs.size();
     }
     span = (double)(last - t0) / 1e9;
-    printf("{\"span_s\": %.6f, \"mean_group\": %.3f, \"lat_us\": [", span, mean_group);
+    printf("{\"span_s\": %.6f, \"mean_group\": %.3f, ", span, mean_group);
+    if (busy && !server) printf("\"device_busy\": %.4f, ", device_ns / 1e9 / span);
+    printf("\"lat_us\": [");
     for (size_t k = 0; k < lat.size(); ++k) printf("%s%.1f", k ? ", " : "", lat[k]);
     printf("]}\n");
     fflush(stdout);
