@@ -124,6 +124,111 @@ struct RoundIO {
   bool has_pointf, has_post;
 };
 
+// First touch of a full round: the item's R values from HBM into registers
+// (zeros for a lane past the end), no arithmetic.
+template <class F, int DIR, int LOG_T, bool CONTIG, int K, int B0>
+__device__ __forceinline__ void load_item(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
+                                          uint32_t pbase) {
+  constexpr int R = 1 << K;
+  if (!li.valid) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = 0;
+    return;
+  }
+  if constexpr (!CONTIG) {
+    // points are adjacent rows: plain global loads, 16-byte vectors when
+    // a thread owns R consecutive points
+    const uint32_t *sp = io.srcp + (li.sbase - io.sbase0) + (uint64_t)pbase * io.sps;
+    if constexpr (B0 == 0 && K >= 2) {
+      const uint4 *s4 = reinterpret_cast<const uint4 *>(sp);
+#pragma unroll
+      for (int i = 0; i < R / 4; ++i) {
+        const uint4 q = s4[i];
+        v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < R; ++i) v[i] = sp[(uint64_t)(i << B0) * io.sps];
+    }
+  } else if (io.srev) {
+    // bitrev_T(pbase | i << B0) = bitrev_T(pbase) | bitrev_K(i) << (T - B0 - K)
+    const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
+    const uint32_t roff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pr * io.srev) * 4u;
+    const uint32_t rstep = io.srev * 4u;
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = bload(io.src, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
+  } else {
+    const uint32_t voff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pbase * io.sps) * 4u;
+    const uint32_t step = (io.sps << B0) * 4u;
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = bload(io.src, voff, i * step);
+  }
+}
+
+// Four-step twist of an inverse Montgomery-field pass, applied to loaded values.
+template <class F, int DIR, int K, int B0, bool LT>
+__device__ __forceinline__ void twist_item(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
+                                           uint32_t pbase) {
+  constexpr int R = 1 << K;
+  if constexpr (F::MONTY && DIR == 1) {
+    if (io.has_twist && li.valid) {
+      if constexpr (LT) mul_lds_table<F, R, B0>(v, io.ltw, pbase);
+      else mul_table<F, R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
+    }
+  }
+}
+
+// Last touch of a full round: forward twist / post factors / scale, then the
+// item's R values from registers to HBM (nothing for a lane past the end).
+template <class F, int DIR, int LOG_T, bool CONTIG, int K, int B0, bool LT>
+__device__ __forceinline__ void store_item(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
+                                           uint32_t pbase) {
+  constexpr int R = 1 << K;
+  if (!li.valid) return;
+  if constexpr (F::MONTY && DIR == 0) {
+    const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
+    if (io.has_twist) mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
+    if (io.has_post) {
+      if constexpr (LT) {
+        // keep the 32 table reads after the butterflies: hoisting them
+        // above the round costs 32 live registers at the peak
+        __builtin_amdgcn_sched_barrier(0);
+        mul_lds_table<F, R, B0>(v, io.lpost, pbase);
+      }
+      else mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
+    }
+  }
+  if (io.has_scale) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if constexpr (F::MONTY) v[i] = F::mul_tw(v[i], make_uint2(io.scale_w, io.scale_q));
+      else v[i] = M31::mul(v[i], io.scale_w);
+    }
+  }
+  if constexpr (!CONTIG) {
+    uint32_t *dp = io.dstp + (li.dbase - io.dbase0) + (uint64_t)pbase * io.dps;
+    if constexpr (B0 == 0 && K >= 2) {
+      uint4 *d4 = reinterpret_cast<uint4 *>(dp);
+#pragma unroll
+      for (int i = 0; i < R / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < R; ++i) dp[(uint64_t)(i << B0) * io.dps] = v[i];
+    }
+  } else if (io.drev) {
+    const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
+    const uint32_t roff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pr * io.drev) * 4u;
+    const uint32_t rstep = io.drev * 4u;
+#pragma unroll
+    for (int i = 0; i < R; ++i) bstore(v[i], io.dst, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
+  } else {
+    const uint32_t voff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pbase * io.dps) * 4u;
+    const uint32_t step = (io.dps << B0) * 4u;
+#pragma unroll
+    for (int i = 0; i < R; ++i) bstore(v[i], io.dst, voff, i * step);
+  }
+}
+
 // Round r of a pass: loads from HBM when FIRST, else from LDS; stores to HBM
 // when LAST, else to LDS.  Items: W << (LOG_T - K).
 template <class F, int DIR, int LOG_T, bool CONTIG, int K, int B0, bool FIRST, bool LAST, bool LT, class TW>
@@ -141,45 +246,8 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     const LaneInfo &li = lanes[lane];
     uint32_t v[R];
     if constexpr (FIRST) {
-      if (li.valid) {
-        const uint32_t voff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pbase * io.sps) * 4u;
-        if constexpr (!CONTIG) {
-          // points are adjacent rows: plain global loads, 16-byte vectors when
-          // a thread owns R consecutive points
-          const uint32_t *sp = io.srcp + (li.sbase - io.sbase0) + (uint64_t)pbase * io.sps;
-          if constexpr (B0 == 0 && K >= 2) {
-            const uint4 *s4 = reinterpret_cast<const uint4 *>(sp);
-#pragma unroll
-            for (int i = 0; i < R / 4; ++i) {
-              const uint4 q = s4[i];
-              v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
-            }
-          } else {
-#pragma unroll
-            for (int i = 0; i < R; ++i) v[i] = sp[(uint64_t)(i << B0) * io.sps];
-          }
-        } else if (io.srev) {
-          // bitrev_T(pbase | i << B0) = bitrev_T(pbase) | bitrev_K(i) << (T - B0 - K)
-          const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
-          const uint32_t roff = (uint32_t)(li.sbase - io.sbase0 + (uint64_t)pr * io.srev) * 4u;
-          const uint32_t rstep = io.srev * 4u;
-#pragma unroll
-          for (int i = 0; i < R; ++i) v[i] = bload(io.src, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
-        } else {
-          const uint32_t step = (io.sps << B0) * 4u;
-#pragma unroll
-          for (int i = 0; i < R; ++i) v[i] = bload(io.src, voff, i * step);
-        }
-        if constexpr (F::MONTY && DIR == 1) {
-          if (io.has_twist) {
-            if constexpr (LT) mul_lds_table<F, R, B0>(v, io.ltw, pbase);
-            else mul_table<F, R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
-          }
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < R; ++i) v[i] = 0;
-      }
+      load_item<F, DIR, LOG_T, CONTIG, K, B0>(v, li, io, pbase);
+      twist_item<F, DIR, K, B0, LT>(v, li, io, pbase);
     } else {
 #pragma unroll
       for (int i = 0; i < R; ++i) v[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
@@ -187,49 +255,7 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     if constexpr (!F::MONTY) tw.hb = li.hb;
     round_regs<F, DIR, K, B0>(v, pbase, tw);
     if constexpr (LAST) {
-      if (!li.valid) continue;
-      if constexpr (F::MONTY && DIR == 0) {
-        const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-        if (io.has_twist) mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
-        if (io.has_post) {
-          if constexpr (LT) {
-            // keep the 32 table reads after the butterflies: hoisting them
-            // above the round costs 32 live registers at the peak
-            __builtin_amdgcn_sched_barrier(0);
-            mul_lds_table<F, R, B0>(v, io.lpost, pbase);
-          }
-          else mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
-        }
-      }
-      if (io.has_scale) {
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          if constexpr (F::MONTY) v[i] = F::mul_tw(v[i], make_uint2(io.scale_w, io.scale_q));
-          else v[i] = M31::mul(v[i], io.scale_w);
-        }
-      }
-      const uint32_t voff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pbase * io.dps) * 4u;
-      if constexpr (!CONTIG) {
-        uint32_t *dp = io.dstp + (li.dbase - io.dbase0) + (uint64_t)pbase * io.dps;
-        if constexpr (B0 == 0 && K >= 2) {
-          uint4 *d4 = reinterpret_cast<uint4 *>(dp);
-#pragma unroll
-          for (int i = 0; i < R / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
-        } else {
-#pragma unroll
-          for (int i = 0; i < R; ++i) dp[(uint64_t)(i << B0) * io.dps] = v[i];
-        }
-      } else if (io.drev) {
-        const uint32_t pr = __brev(pbase) >> (32 - LOG_T);
-        const uint32_t roff = (uint32_t)(li.dbase - io.dbase0 + (uint64_t)pr * io.drev) * 4u;
-        const uint32_t rstep = io.drev * 4u;
-#pragma unroll
-        for (int i = 0; i < R; ++i) bstore(v[i], io.dst, roff, (brev_c<K>(i) << (LOG_T - B0 - K)) * rstep);
-      } else {
-        const uint32_t step = (io.dps << B0) * 4u;
-#pragma unroll
-        for (int i = 0; i < R; ++i) bstore(v[i], io.dst, voff, i * step);
-      }
+      store_item<F, DIR, LOG_T, CONTIG, K, B0, LT>(v, li, io, pbase);
     } else {
 #pragma unroll
       for (int i = 0; i < R; ++i) tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)] = v[i];
@@ -250,7 +276,8 @@ __device__ __forceinline__ void rounds(uint32_t *tile, uint32_t log_w, const Lan
   }
 }
 
-// LDS: [LaneInfo x W][root table T x uint2 (Montgomery fields)][tile W x (T + T/32)]
+// LDS bytes of a tuned pass: lane records (W x LaneInfo), root table (T x uint2,
+// Montgomery fields) and tile (W x (T + T/32) words)
 __host__ __device__ constexpr size_t lds_bytes(int field, int log_t, uint32_t log_w) {
   return (kLaneInfoBytes << log_w) + (field != 0 ? ((size_t)8 << log_t) : 0) +
          ((((size_t)1 << log_t) + ((size_t)1 << log_t) / 32) << log_w) * 4;
@@ -340,10 +367,13 @@ __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, L
 template <class F, int DIR, int LOG_T, bool CONTIG, int LW = -1>
 __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
-  uint2 *root = reinterpret_cast<uint2 *>(smem + (kLaneInfoBytes << a.log_w));
-  uint32_t *tile = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(root) +
-                                                (F::MONTY ? ((size_t)8 << LOG_T) : 0));
+  // LDS: [tile][root table][lane records]: with the tile first, more of an
+  // item's exchange slots are immediate offsets (< 64 KiB) from one base
+  const uint32_t log_w = LW >= 0 ? (uint32_t)LW : a.log_w;
+  uint32_t *tile = reinterpret_cast<uint32_t *>(smem);
+  uint2 *root = reinterpret_cast<uint2 *>(smem + ((((size_t)1 << LOG_T) + ((size_t)1 << LOG_T) / 32) << log_w) * 4);
+  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(reinterpret_cast<unsigned char *>(root) +
+                                                 (F::MONTY ? ((size_t)8 << LOG_T) : 0));
   if constexpr (F::MONTY) load_root<LOG_T>(a, root);
   fast_tile<F, DIR, LOG_T, CONTIG, LOG_T, LW>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
 }

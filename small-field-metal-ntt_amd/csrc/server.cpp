@@ -4,10 +4,12 @@
 //  - submit() validates and enqueues (FIFO), bounded by max_queue;
 //  - max_inflight worker threads each own one HIP stream.  A free worker takes
 //    the oldest queued job it may run, plus the queued jobs it can fuse with
-//    it (continuous batching).  Large jobs (more than kSmallJobElems output
-//    elements) run one at a time, as they fill the device alone; worker 0 runs
-//    small jobs only when there are several workers, so small jobs never wait
-//    behind large ones.  Fusion:
+//    it (continuous batching); an idle worker polls for work for kPollNs
+//    before it sleeps.  Large jobs (more than kSmallJobElems output elements)
+//    run one at a time, as they fill the device alone; worker 0 runs small
+//    jobs only when there are several workers, so small jobs never wait
+//    behind large ones.  A large host-memory job runs alone, in overlapped
+//    column chunks.  Fusion:
 //      NTT jobs of any field, direction and size run together as one
 //      many-job launch per pass;
 //      LDE jobs of one (log_n, added_bits, shift) run as one batched LDE over
@@ -24,6 +26,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -45,6 +48,9 @@ namespace {
 constexpr uint64_t kSmallJobElems = 1ull << 22;
 // one fused group moves at most this many elements (bounds its latency)
 constexpr uint64_t kGroupElems = 1ull << 27;
+// an idle worker polls this long for new work before it sleeps: waking a
+// sleeping thread costs tens of microseconds, as much as a small job
+constexpr uint64_t kPollNs = 100000;
 
 struct Job {
   uint64_t ticket = 0;
@@ -85,6 +91,8 @@ struct sfn_server {
   size_t reserved = 0;  // device bytes of every worker's staging buffers (held or being allocated)
   uint32_t running = 0;
   bool large_running = false;  // a group of large jobs is on the device
+  bool starved = false;        // a host-memory job waits for staging budget
+  std::atomic<uint64_t> epoch{0};  // bumped by every submit and every finished group
   uint32_t waiting = 0;        // threads inside sfn_server_wait
   bool stop = false;
   std::vector<std::thread> workers;
@@ -138,12 +146,21 @@ bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, si
     return others + std::max(stg.in.bytes, in) + std::max(stg.out.bytes, out) <= cfg.mem_budget_bytes;
   };
   const bool host = !head->req.device_ptrs;
-  if (host && !fits(head->in_bytes, head->out_bytes)) return false;  // wait for staging to free up
+  if (host && !fits(head->in_bytes, head->out_bytes)) {
+    // wait for staging to free up: idle workers give theirs back
+    if (!starved) cv_work.notify_all();
+    starved = true;
+    return false;
+  }
+  starved = false;
   group.assign(1, head);
   need_in = head->in_bytes;
   need_out = head->out_bytes;
   uint64_t elems = head->out_elems(), cols = head->req.batch;
-  for (auto it = std::next(head_it); it != queue.end() && group.size() < cfg.max_group; ++it) {
+  // a large host-memory job runs alone, in column chunks whose copies overlap
+  // its kernels; fused, its upload, kernels and download would run in turn
+  const uint32_t max_group = host && !head->small() ? 1u : cfg.max_group;
+  for (auto it = std::next(head_it); it != queue.end() && group.size() < max_group; ++it) {
     Job *j = *it;
     if (!j->fuses_with(*head) || j->small() != head->small()) continue;
     // device-memory LDE jobs are gathered into staging only while launches dominate
@@ -308,13 +325,23 @@ void sfn_server::worker_main(int idx) {
     bool staged = false, shed = false, large = false;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] {
+      auto ready = [&] {
         if (stop) return true;
         if (pick(idx, stg, group, need_in, need_out, staged)) return true;
-        // an idle worker holding staging gives it back while jobs wait for budget
-        shed = held > 0 && !queue.empty();
+        // an idle worker holding staging gives it back while a job waits for
+        // budget (keeping it otherwise: a hipFree waits for the whole device)
+        shed = held > 0 && starved;
         return shed;
-      });
+      };
+      if (!ready()) {
+        const uint64_t seen = epoch.load(std::memory_order_acquire);
+        lk.unlock();
+        for (const uint64_t until = sfn_now_ns() + kPollNs;
+             epoch.load(std::memory_order_acquire) == seen && sfn_now_ns() < until;)
+          std::this_thread::yield();
+        lk.lock();
+        cv_work.wait(lk, ready);
+      }
       if (group.empty() && !shed) break;  // stopping
       if (!group.empty()) {
         if (staged) {
@@ -360,6 +387,7 @@ void sfn_server::worker_main(int idx) {
       if (large) large_running = false;
       --running;
     }
+    epoch.fetch_add(1, std::memory_order_release);
     cv_done.notify_all();
     cv_work.notify_all();
   }
@@ -451,6 +479,7 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
   *ticket = job->ticket;
   srv->queue.push_back(job.get());
   srv->jobs.emplace(job->ticket, std::move(job));
+  srv->epoch.fetch_add(1, std::memory_order_release);
   srv->cv_work.notify_one();
   return SFN_OK;
 }

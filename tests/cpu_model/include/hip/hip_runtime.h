@@ -19,6 +19,7 @@
 //  - streams, events and graphs run synchronously on the calling thread;
 //    kernel launches are serialised;
 //  - copies whose device side is not inside one hipMalloc allocation fail.
+// SFNTT_MODEL_CUS=<n> reports n compute units instead of 256.
 // Fault injection: SFNTT_MODEL_FAIL="<what>:<n>[,<what>:<n>...]" makes the
 // n-th (1-based) call of `what` fail, what in {malloc, host_malloc, launch,
 // memcpy}.  sfn_model_* functions (below) read and reset counters.

@@ -315,7 +315,10 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int device) {
   memset(p, 0, sizeof(*p));
   strcpy(p->name, "CPU model of gfx950");
   strcpy(p->gcnArchName, "gfx950:sramecc+:xnack-");
-  p->multiProcessorCount = 256;
+  // SFNTT_MODEL_CUS: fewer compute units, so persistent launches (one
+  // workgroup per resident slot) loop over several tiles at small sizes
+  const char *cus = getenv("SFNTT_MODEL_CUS");
+  p->multiProcessorCount = cus && atoi(cus) > 0 ? atoi(cus) : 256;
   p->maxSharedMemoryPerMultiProcessor = kLdsBytes;
   p->totalGlobalMem = (size_t)288 << 30;
   p->clockRate = 2400000;

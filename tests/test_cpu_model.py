@@ -39,10 +39,21 @@ SUITES = ["ntt", "lde", "plans", "server", "faults", "cache", "noalloc",
           "schedules_full" if os.environ.get("SFNTT_MODEL_FULL") == "1" else "schedules"]
 
 
-@pytest.mark.parametrize("suite", SUITES)
-def test_cpu_model_suite(model_env, suite):
-    r = subprocess.run([sys.executable, os.path.join(MODEL, "run_model.py"), suite], env=model_env,
+def _run_suite(env, suite):
+    r = subprocess.run([sys.executable, os.path.join(MODEL, "run_model.py"), suite], env=env,
                        capture_output=True, text=True, timeout=1800)
     tail = "\n".join((r.stdout + r.stderr).splitlines()[-40:])
     assert r.returncode == 0, f"suite {suite} failed (rc {r.returncode}):\n{tail}"
     assert f"suite {suite}: ok" in r.stdout, tail
+
+
+@pytest.mark.parametrize("suite", SUITES)
+def test_cpu_model_suite(model_env, suite):
+    _run_suite(model_env, suite)
+
+
+@pytest.mark.parametrize("suite", ["plans", "server"])
+def test_cpu_model_few_compute_units(model_env, suite):
+    """Four compute units: the many-job launches, whose grid is sized to the
+    device, run several tiles per workgroup."""
+    _run_suite(dict(model_env, SFNTT_MODEL_CUS="4"), suite)
