@@ -256,10 +256,11 @@ typedef struct {
 } sfn_request;
 
 typedef struct {
-    size_t mem_budget_bytes;  /* device bytes the server may hold for job buffers (0 = 8 GiB) */
-    uint32_t max_inflight;    /* worker streams, i.e. launch sequences on the device at once (0 = 4) */
+    size_t mem_budget_bytes;  /* device bytes the server may hold for job buffers
+                                 (0 = a quarter of the device's memory, at least 8 GiB) */
+    uint32_t max_inflight;    /* worker streams, i.e. launch sequences on the device at once (0 = 6) */
     uint32_t max_queue;       /* admission queue bound (0 = 4096) */
-    uint32_t max_group;       /* most queued same-shape jobs fused into one launch (0 = 64; 1 = no grouping) */
+    uint32_t max_group;       /* most queued jobs fused into one launch set (0 = 128; 1 = no grouping) */
     uint32_t max_group_cols;  /* column cap of one fused launch (0 = 512) */
 } sfn_server_config;
 

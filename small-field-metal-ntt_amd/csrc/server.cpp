@@ -5,11 +5,13 @@
 //  - max_inflight worker threads each own one HIP stream.  A free worker takes
 //    the oldest queued job it may run, plus the queued jobs it can fuse with
 //    it (continuous batching); an idle worker polls for work for kPollNs
-//    before it sleeps.  Large jobs (more than kSmallJobElems output elements)
-//    run one at a time, as they fill the device alone; worker 0 runs small
-//    jobs only when there are several workers, so small jobs never wait
-//    behind large ones.  A large host-memory job runs alone, in overlapped
-//    column chunks.  Fusion:
+//    before it sleeps.  Large device-memory jobs (more than kSmallJobElems
+//    output elements) run at most kLargeDevGroups groups at a time, as they
+//    (nearly) fill the device; large host-memory jobs, bound by their PCIe
+//    copies, run two at a time,
+//    each alone in overlapped column chunks; worker 0 runs small jobs only
+//    when there are several workers, so small jobs never wait behind large
+//    ones.  Fusion:
 //      NTT jobs of any field, direction and size run together as one
 //      many-job launch per pass;
 //      LDE jobs of one (log_n, added_bits, shift) run as one batched LDE over
@@ -46,6 +48,14 @@ namespace {
 
 // jobs up to this many output elements are "small" (the dedicated worker's lane)
 constexpr uint64_t kSmallJobElems = 1ull << 22;
+// large device-memory groups running at once: two, so that one whose grid
+// does not fill the 256 CUs (a few hundred workgroups) leaves the rest to the
+// other, while small jobs still find free streams
+constexpr uint32_t kLargeDevGroups = 2;
+// large host-memory jobs running at once: each is bound by its PCIe copies and
+// leaves most of the device idle, and two keep both link directions busy
+// (one uploading while the other downloads)
+constexpr uint32_t kLargeHostJobs = 2;
 // one fused group moves at most this many elements (bounds its latency)
 constexpr uint64_t kGroupElems = 1ull << 27;
 // an idle worker polls this long for new work before it sleeps: waking a
@@ -90,7 +100,8 @@ struct sfn_server {
   uint64_t next_ticket = 1;
   size_t reserved = 0;  // device bytes of every worker's staging buffers (held or being allocated)
   uint32_t running = 0;
-  bool large_running = false;  // a group of large jobs is on the device
+  uint32_t large_dev = 0;   // large device-memory groups running (at most kLargeDevGroups)
+  uint32_t large_host = 0;  // large host-memory jobs running (at most kLargeHostJobs)
   bool starved = false;        // a host-memory job waits for staging budget
   std::atomic<uint64_t> epoch{0};  // bumped by every submit and every finished group
   uint32_t waiting = 0;        // threads inside sfn_server_wait
@@ -134,10 +145,16 @@ void release(sfn::DevBuf &b) {
 bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, size_t &need_in, size_t &need_out,
                       bool &staged) {
   const bool small_lane = idx == 0 && cfg.max_inflight > 1;
-  // the oldest job this worker may start: small ones always, a large one when
-  // this is not the small lane and no large group is running
+  // the oldest job this worker may start: small ones always; a large one when
+  // this is not the small lane and fewer than kLargeDevGroups (device memory)
+  // or kLargeHostJobs (host memory) large ones run
+  auto may_start = [&](const Job *j) {
+    if (j->small()) return true;
+    if (small_lane) return false;
+    return j->req.device_ptrs ? large_dev < kLargeDevGroups : large_host < kLargeHostJobs;
+  };
   auto head_it = queue.begin();
-  while (head_it != queue.end() && !(*head_it)->small() && (small_lane || large_running)) ++head_it;
+  while (head_it != queue.end() && !may_start(*head_it)) ++head_it;
   if (head_it == queue.end()) return false;
   Job *head = *head_it;
   const size_t others = reserved - stg.held();  // the other workers' staging
@@ -322,7 +339,8 @@ void sfn_server::worker_main(int idx) {
   for (;;) {
     std::vector<Job *> group;
     size_t need_in = 0, need_out = 0;
-    bool staged = false, shed = false, large = false;
+    bool staged = false, shed = false;
+    uint32_t *large = nullptr;  // the large-job counter this group holds
     {
       std::unique_lock<std::mutex> lk(mu);
       auto ready = [&] {
@@ -350,8 +368,10 @@ void sfn_server::worker_main(int idx) {
           held = grown;
         }
         ++running;
-        large = !group[0]->small();
-        if (large) large_running = true;
+        if (!group[0]->small()) {
+          large = group[0]->req.device_ptrs ? &large_dev : &large_host;
+          ++*large;
+        }
         const uint64_t t = sfn_now_ns();
         for (Job *j : group) {
           j->start_ns = t;
@@ -384,7 +404,7 @@ void sfn_server::worker_main(int idx) {
         j->status = st;
         j->done = true;
       }
-      if (large) large_running = false;
+      if (large) --*large;
       --running;
     }
     epoch.fetch_add(1, std::memory_order_release);
@@ -428,11 +448,13 @@ sfn_status sfn_server_create(sfn_ctx *cx, const sfn_server_config *cfg, sfn_serv
   auto srv = std::make_unique<sfn_server>();
   srv->ctx = &cx->impl;
   if (cfg) srv->cfg = *cfg;
-  if (!srv->cfg.mem_budget_bytes) srv->cfg.mem_budget_bytes = 8ull << 30;
-  if (!srv->cfg.max_inflight) srv->cfg.max_inflight = 4;
+  // default budget: a quarter of the device's memory (72 GB on a 288 GB MI355X)
+  if (!srv->cfg.mem_budget_bytes)
+    srv->cfg.mem_budget_bytes = std::max<size_t>(8ull << 30, cx->impl.props.totalGlobalMem / 4);
+  if (!srv->cfg.max_inflight) srv->cfg.max_inflight = 6;
   if (srv->cfg.max_inflight > 32) srv->cfg.max_inflight = 32;
   if (!srv->cfg.max_queue) srv->cfg.max_queue = 4096;
-  if (!srv->cfg.max_group) srv->cfg.max_group = 64;
+  if (!srv->cfg.max_group) srv->cfg.max_group = 128;
   if (!srv->cfg.max_group_cols) srv->cfg.max_group_cols = 512;
   for (uint32_t i = 0; i < srv->cfg.max_inflight; ++i)
     srv->workers.emplace_back(&sfn_server::worker_main, srv.get(), (int)i);

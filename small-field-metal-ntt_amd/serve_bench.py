@@ -112,8 +112,8 @@ def main():
     ap.add_argument("--rates", default="")
     ap.add_argument("--p99-ms", default="20,50,100,200", help="p99 targets (ms), comma separated")
     ap.add_argument("--scale", type=int, default=0, help="shrink every log_n by this much (quick runs)")
-    ap.add_argument("--inflight", type=int, default=4)
-    ap.add_argument("--group", type=int, default=64, help="most jobs the server fuses into one launch set")
+    ap.add_argument("--inflight", type=int, default=6, help="server worker streams")
+    ap.add_argument("--group", type=int, default=128, help="most jobs the server fuses into one launch set")
     ap.add_argument("--modes", default="device,host")
     ap.add_argument("--write-trace", default="",
                     help="only write the trace (every job arriving at once, closed loop) to this file, e.g. to "
