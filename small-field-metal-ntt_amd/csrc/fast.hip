@@ -263,16 +263,18 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
   }
 }
 
-template <class F, int DIR, int LOG_T, bool CONTIG, int r, int rend, bool LT = false, class TW>
+// Rounds [r, rend) of a pass whose rounds are KRT layers each (the partial
+// round as the schedule places it).
+template <class F, int DIR, int LOG_T, bool CONTIG, int r, int rend, bool LT = false, int KRT = KR, class TW>
 __device__ __forceinline__ void rounds(uint32_t *tile, uint32_t log_w, const LaneInfo *lanes, const TW &tw,
                                        const RoundIO &io) {
-  using S = Sched<LOG_T, KR>;
+  using S = Sched<LOG_T, KRT>;
   if constexpr (r < rend) {
     constexpr int K = S::template k<DIR, r>();
     constexpr int B0 = S::template b0<DIR, r>();
     run_round<F, DIR, LOG_T, CONTIG, K, B0, r == 0, r == S::NR - 1, LT>(tile, log_w, lanes, tw, io);
     if constexpr (r + 1 < rend) __syncthreads();
-    rounds<F, DIR, LOG_T, CONTIG, r + 1, rend, LT>(tile, log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, r + 1, rend, LT, KRT>(tile, log_w, lanes, tw, io);
   }
 }
 
@@ -346,7 +348,9 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
 // LW >= 0: the launch's lanes per workgroup are 2^LW (a.log_w == LW), known at
 // compile time, so every LDS slot index of the rounds folds to an immediate
 // offset from one per-item base.
-template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG, int LW = -1>
+// KRT: layers per register round (5; 6 gives a 2^11-point pass two rounds,
+// 6 + 5, instead of 5 + 5 + 1, for 64 values per item in the 6-layer round).
+template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG, int LW = -1, int KRT = KR>
 __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, LaneInfo *lanes, const uint2 *root,
                                           uint32_t *tile) {
   static_assert(ROOT_LOG >= LOG_T, "root table too small");
@@ -357,10 +361,10 @@ __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, L
   if constexpr (F::MONTY) {
     RootTw<ROOT_LOG, DIR> tw;
     tw.tab = root;
-    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KRT>::NR, false, KRT>(tile, log_w, lanes, tw, io);
   } else {
     const auto tw = make_tw<F, DIR, LOG_T>(a, nullptr);
-    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KR>::NR>(tile, log_w, lanes, tw, io);
+    rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KRT>::NR, false, KRT>(tile, log_w, lanes, tw, io);
   }
 }
 
@@ -631,8 +635,11 @@ __device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, ui
   switch (log_t) {
 #define SFN_CASE(L) \
   case L: fast_tile<F, DIR, L, CONTIG, kMaxLogTile>(a, tile, lanes, root, tile_lds); break;
-    SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10) SFN_CASE(11)
+    SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10)
 #undef SFN_CASE
+    // 2^11 points in two rounds (6 + 5 layers), so the largest columns of a
+    // launch set take two passes of 11 layers
+    case 11: fast_tile<F, DIR, 11, CONTIG, kMaxLogTile, -1, 6>(a, tile, lanes, root, tile_lds); break;
     default: break;
   }
 }

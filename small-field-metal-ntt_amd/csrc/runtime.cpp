@@ -1132,9 +1132,10 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
     if (m.log_n == 0) continue;
     Schedule sc = ctx->schedule_for(m.field, m.log_n, m.batch, SFN_LAYOUT_SOA);
     // Many jobs fill the device together, so passes may be large: up to 2^11
-    // points, except that two 11-layer passes (three register rounds each)
-    // lose to three shorter ones (measured on 2^22 columns).  A tuned entry
-    // keeps its pass size; the radix is always the tuned kernel's.
+    // points, except that two 11-layer passes lose to three shorter ones on
+    // 2^22 columns (measured: the 2^11-point tile holds only 4 lanes, so its
+    // strided pass moves 16-byte row pieces).  A tuned entry keeps its pass
+    // size; the radix is always the tuned kernel's.
     if (!ctx->has_tuning(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}))
       sc.max_log_t = m.log_n > 2 * (kMaxLogTile - 1) + 1 ? kMaxLogTile - 1 : kMaxLogTile;
     sc.logr = 5;
