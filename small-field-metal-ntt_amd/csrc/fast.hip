@@ -77,11 +77,26 @@ __device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *src, 
 // Montgomery-field twiddles from the LDS root table (Shoup pairs).
 template <int LOG_T, int DIR>
 struct RootTw {
+  static constexpr bool kNeg = false;
   const uint2 *tab;
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
     uint32_t e = pos << (LOG_T - t - 1);
     if (DIR == 1) e = (0u - e) & ((1u << LOG_T) - 1);
     return tab[e];
+  }
+};
+
+// Half root table g_T^e, e <= T/2 (T/2 + 1 Shoup pairs).  Layer t, position j
+// uses e = j * 2^(LOG_T-t-1) < T/2.  The inverse needs g^-e = -g^(T/2-e)
+// (g^(T/2) = -1), so it returns g^(T/2-e) and its CT butterflies take the
+// twiddle as negated (kNeg); e = 0 gives -g^(T/2) = 1 as it should.
+template <int LOG_T, int DIR>
+struct HalfRootTw {
+  static constexpr bool kNeg = DIR == 1;
+  const uint2 *tab;
+  __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
+    const uint32_t e = pos << (LOG_T - t - 1);
+    return tab[DIR == 1 ? (1u << (LOG_T - 1)) - e : e];
   }
 };
 
@@ -307,6 +322,13 @@ __device__ __forceinline__ void load_root(const PassArgs &a, uint2 *tab) {
   for (uint32_t e = threadIdx.x; e < (1u << LOG_T); e += blockDim.x) tab[e] = root[e << (kMaxLogTile - LOG_T)];
 }
 
+// The half table g_T^e, e <= T/2, from the same root table.
+template <int LOG_T>
+__device__ __forceinline__ void load_root_half(const PassArgs &a, uint2 *tab) {
+  const uint2 *root = reinterpret_cast<const uint2 *>(a.tw);
+  for (uint32_t e = threadIdx.x; e <= (1u << (LOG_T - 1)); e += blockDim.x) tab[e] = root[e << (kMaxLogTile - LOG_T)];
+}
+
 // Common workgroup set-up: lane records in LDS, buffer resources rebased at
 // lane 0 (decoded by every thread, so the bases stay scalar).
 template <int LOG_T>
@@ -395,8 +417,24 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
 // CONTIG: lanes adjacent in memory (the middle layers of a multi-pass LDE);
 // otherwise every lane is one whole column of points adjacent in memory (an
 // LDE of at most 2^11 rows, SoA: lo == 0, 16-byte vector loads and stores).
+// LDS of the middle pass: the tile (W x (T + T/32) words), half root table
+// (T/2 + 1 Shoup pairs, padded to 16 bytes), LT tables (2T words) and lane records.
+__host__ __device__ constexpr size_t mid_root_bytes(int log_t) {
+  return ((((size_t)1 << (log_t - 1)) + 1) * 8 + 15) & ~(size_t)15;
+}
+// tile bytes padded to 16, so the tables after it take 16-byte accesses
+__host__ __device__ constexpr size_t mid_tile_bytes(int log_t, uint32_t log_w) {
+  return ((((((size_t)1 << log_t) + ((size_t)1 << log_t) / 32) << log_w) * 4) + 15) & ~(size_t)15;
+}
+__host__ __device__ constexpr size_t mid_lds_bytes(int log_t, uint32_t log_w, bool lt) {
+  return mid_tile_bytes(log_t, log_w) + mid_root_bytes(log_t) + (lt ? ((size_t)8 << log_t) : 0) +
+         (kLaneInfoBytes << log_w);
+}
+
+// 16 lanes (512 threads, two workgroups per CU): at most 128 VGPRs, so four
+// waves per SIMD; otherwise three.
 template <int LOG_T, bool LT, int LW = -1, bool CONTIG = true>
-__global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
+__global__ __launch_bounds__(512, LW == 4 ? 4 : 3) void k_mid_fast(const PassArgs a) {
   using F = BabyBear;
   using S = Sched<LOG_T, KR>;
   static_assert(S::NR >= 2, "middle pass needs two rounds");
@@ -408,13 +446,16 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
   constexpr uint32_t T = 1u << LOG_T;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t log_w = LW >= 0 ? (uint32_t)LW : a.log_w;  // a.log_w == LW when LW >= 0
-  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem);
-  unsigned char *tabp = smem + (kLaneInfoBytes << log_w);
-  uint32_t *tabs = reinterpret_cast<uint32_t *>(tabp + ((size_t)8 << LOG_T));
-  uint32_t *tile = tabs + (LT ? 2 * T : 0);
+  // LDS: [tile][half root table][LT tables][lane records]: with the tile
+  // first, every exchange slot of an item is an immediate offset (< 64 KiB)
+  // from one per-thread base, so no address registers are held per slot
+  uint32_t *tile = reinterpret_cast<uint32_t *>(smem);
+  unsigned char *tabp = smem + mid_tile_bytes(LOG_T, log_w);
+  uint32_t *tabs = reinterpret_cast<uint32_t *>(tabp + mid_root_bytes(LOG_T));
+  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(tabs + (LT ? 2 * T : 0));
   RoundIO io;
   setup<LOG_T>(a, lanes, io, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log));
-  load_root<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
+  load_root_half<LOG_T>(a, reinterpret_cast<uint2 *>(tabp));
   // LT tables: [point factors of the current block: T words][post of the
   // current block: T words]; the post half holds the inverse twist first.
   if constexpr (LT) {
@@ -423,8 +464,8 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
     io.ltw = tabs + T;
   }
   __syncthreads();
-  RootTw<LOG_T, 1> twi;
-  RootTw<LOG_T, 0> twf;
+  HalfRootTw<LOG_T, 1> twi;
+  HalfRootTw<LOG_T, 0> twf;
   twi.tab = twf.tab = reinterpret_cast<const uint2 *>(tabp);
   // inverse rounds 0 .. NR-2 (round 0 loads from HBM with the inverse twist)
   rounds<F, 1, LOG_T, CONTIG, 0, S::NR - 1, LT>(tile, log_w, lanes, twi, io);
@@ -469,8 +510,15 @@ __global__ __launch_bounds__(512, 3) void k_mid_fast(const PassArgs a) {
     }
     round_regs<F, 0, K, B0>(v, pbase, twf);
     __syncthreads();  // every thread is done reading the tile (and the tables in use)
+    {
+      // the same slots as lidx, from a lane and pbase re-derived in the loop
+      // with their known-zero bits (W - 1 and the item's own point bits), so
+      // the 32 slots are immediate offsets from one base
+      const uint32_t ln = loop_local(lane) & (W - 1);
+      const uint32_t pb = loop_local(pbase) & ((T - 1) & ~((uint32_t)(R - 1) << B0));
 #pragma unroll
-    for (int i = 0; i < R; ++i) tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)] = v[i];
+      for (int i = 0; i < R; ++i) tile[lidx<LOG_T, CONTIG>(ln, pb | (i << B0), log_w)] = v[i];
+    }
     if constexpr (LT) {
       if (pre) {
         if (e4 < T) {
@@ -579,13 +627,18 @@ static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t l
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 3, CONTIG>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+    (void)hipFuncSetAttribute((const void *)fast::k_mid_fast<LOG_T, LT, 4, CONTIG>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
     return true;
   }();
   (void)configured;
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
-  // eight adjacent lanes (the default middle-pass width): lane count fixed at compile time
+  // eight or sixteen adjacent lanes (2^10-point tiles): lane count fixed at compile time
   if (CONTIG && LOG_T == 10 && a.log_w == 3)
     hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, 3, CONTIG>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds,
+                       s, a);
+  else if (CONTIG && LOG_T == 10 && a.log_w == 4)
+    hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, 4, CONTIG>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds,
                        s, a);
   else
     hipLaunchKernelGGL((fast::k_mid_fast<LOG_T, LT, -1, CONTIG>), dim3((uint32_t)tiles), dim3((uint32_t)threads),
@@ -597,11 +650,11 @@ static hipError_t launch_mid_fast_lt(const PassArgs &a, uint64_t tiles, size_t l
 // and the extra 2T words fit; otherwise per-lane global table reads.
 template <int LOG_T>
 static hipError_t launch_mid_fast_t(bool contig, const PassArgs &a, uint64_t tiles, hipStream_t s) {
-  const size_t lds = fast::lds_bytes(1, LOG_T, a.log_w);
+  const size_t lds = fast::mid_lds_bytes(LOG_T, a.log_w, false);
   const uint64_t threads = (uint64_t)1 << (a.log_w + LOG_T - fast::KR);
   if (lds > kMaxLdsBytes || threads > 512 || tiles > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
   if (!contig) return launch_mid_fast_lt<LOG_T, false, false>(a, tiles, lds, s);
-  const size_t lds_lt = lds + ((size_t)8 << LOG_T);
+  const size_t lds_lt = fast::mid_lds_bytes(LOG_T, a.log_w, true);
   if (a.tab_low_major && lds_lt <= kMaxLdsBytes) return launch_mid_fast_lt<LOG_T, true, true>(a, tiles, lds_lt, s);
   return launch_mid_fast_lt<LOG_T, false, true>(a, tiles, lds, s);
 }

@@ -102,6 +102,16 @@ __device__ __forceinline__ void bfly_ct(uint32_t &x, uint32_t &y, typename F::Tw
   y = F::sub(a, t);
 }
 
+// CT with the negated twiddle: given w' = -w, (x, y) -> (x - w'y, x + w'y)
+// = (x + wy, x - wy).
+template <class F>
+__device__ __forceinline__ void bfly_ct_neg(uint32_t &x, uint32_t &y, typename F::Tw wn) {
+  uint32_t t = F::mul_tw(y, wn);
+  uint32_t a = x;
+  x = F::sub(a, t);
+  y = F::add(a, t);
+}
+
 template <class F>
 __device__ __forceinline__ void bfly_gs(uint32_t &x, uint32_t &y, typename F::Tw w) {
   uint32_t a = x, b = y;

@@ -80,12 +80,14 @@ struct TwCtx;
 
 template <class F>
 struct TwCtx<F, true> {
+  static constexpr bool kNeg = false;
   const uint2 *stw;  // LDS
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const { return stw[(1u << t) - 1 + pos]; }
 };
 
 template <>
 struct TwCtx<M31, false> {
+  static constexpr bool kNeg = false;
   const uint32_t *tbl;  // global, whole transform
   uint32_t n_log, lo, hi;
   uint64_t hb;
@@ -98,6 +100,8 @@ struct TwCtx<M31, false> {
 };
 
 // One register round: K layers on v[0..2^K) holding points pbase | (i << B0).
+// A fetcher with kNeg returns the negated twiddle -w for the inverse (CT)
+// butterflies, which then compute (x - (-w)y, x + (-w)y) = (x + wy, x - wy).
 template <class F, int DIR, int K, int B0, class TW>
 __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase, const TW &tw) {
   constexpr int R = 1 << K;
@@ -150,6 +154,7 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
             }
           }
           if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          else if constexpr (TW::kNeg) bfly_ct_neg<F>(v[i], v[i + (1 << j)], w);
           else bfly_ct<F>(v[i], v[i + (1 << j)], w);
         }
       }
@@ -168,6 +173,15 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t g, uint32_t grid, uint32_t
   if (s == 0 || g >= grid - grid % gs) return g;
   const uint32_t r = g & (gs - 1);
   return (g - r) + ((r & 7u) << s) + (r >> 3);
+}
+
+// Returns x unchanged as a value the compiler cannot move out of the
+// enclosing loop (an empty asm statement).  Address arithmetic on it then stays
+// next to its loads and stores and folds into their immediate offsets, where
+// loop-invariant code motion would otherwise hold one register per address.
+__device__ __forceinline__ uint32_t loop_local(uint32_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
 }
 
 template <int K, int B0>

@@ -502,9 +502,10 @@ Schedule Context::schedule_for_batch(int field, uint32_t log_n, uint32_t batch, 
 // 1.1-4.8x faster than the batch rule, whose small tiles split these sizes
 // into five or more launches); SoA 2^9..2^10 prefers radix-8 outer passes, and
 // 2^8 (but row-major batches of >= 8 columns) and smaller keep the batch rule.
-// At every size the middle pass takes as many lanes as its LDS allows three
-// workgroups per CU: 32 for tiles of <= 2^8 points, 16 for 2^9, 8 for 2^10 and
-// up (larger calls 1.07-2.4x faster, the 2^20 x 256 LDE unchanged).
+// The middle pass takes 32 lanes for tiles of <= 2^8 points and 16 for 2^9
+// and 2^10 (larger calls 1.07-2.4x faster than 8); at 2^10 points 16 lanes
+// are two workgroups of 8 waves per CU at <= 128 VGPRs, four waves per SIMD
+// (2^20 x 256 row-major: middle pass 1797 -> 1492 us against 8 lanes).
 Schedule Context::schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) const {
   const uint64_t elems = (uint64_t)batch << log_n;
   const bool rm = layout == SFN_LAYOUT_ROW_MAJOR;
@@ -527,7 +528,7 @@ Schedule Context::schedule_for_lde(uint32_t log_n, uint32_t batch, int layout) c
   }
   const uint32_t mlt = std::min<uint32_t>(s.max_log_t, kMaxLogTile);
   const uint32_t mid_t = std::min<uint32_t>(log_n <= mlt ? log_n : (log_n + 1) / 2, mlt);
-  s.mid_log_w = mid_t >= 10 ? 3 : mid_t == 9 ? 4 : 5;
+  s.mid_log_w = mid_t >= 10 ? 4 : mid_t == 9 ? 4 : 5;
   return s;
 }
 

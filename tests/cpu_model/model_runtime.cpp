@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <set>
 #include <string>
 #include <thread>
@@ -79,6 +80,27 @@ void parse_faults(const char *spec) {
 struct FaultInit {
   FaultInit() { parse_faults(getenv("SFNTT_MODEL_FAIL")); }
 } g_fault_init;
+
+// ---- fiber order ----
+// SFNTT_MODEL_ORDER=reverse runs a workgroup's threads last to first between
+// barriers; shuffle[:seed] draws a new order for every sweep.  A kernel whose
+// threads exchange LDS data without a barrier in between, or overwrite data
+// another thread still has to read, then gives different results than in
+// thread order.
+struct FiberOrder {
+  int mode = 0;  // 0 thread order, 1 reverse, 2 shuffle
+  uint64_t seed = 1;
+  FiberOrder() {
+    const char *o = getenv("SFNTT_MODEL_ORDER");
+    if (!o) return;
+    std::string s(o);
+    if (s == "reverse") mode = 1;
+    if (s.rfind("shuffle", 0) == 0) {
+      mode = 2;
+      if (s.size() > 8 && s[7] == ':') seed = strtoull(s.c_str() + 8, nullptr, 10);
+    }
+  }
+} g_fiber_order;
 
 // ---- device memory ----
 std::mutex g_mem_mu;
@@ -235,10 +257,15 @@ hipError_t run_grid(const void *kernel, dim3 grid, dim3 block, size_t lds, sfn_m
       makecontext(&f.ctx, fiber_entry, 0);
     }
     // sweep until every fiber has returned; each sweep takes every fiber
-    // still running to its next barrier
+    // still running to its next barrier, in the order SFNTT_MODEL_ORDER asks
+    std::vector<uint32_t> order(nthreads);
+    std::mt19937_64 rng(g_fiber_order.seed * 0x9E3779B97F4A7C15ull + g_launches.load() * 1315423911ull + g);
     for (bool any = true; any;) {
       any = false;
-      for (uint32_t t = 0; t < nthreads; ++t) {
+      for (uint32_t t = 0; t < nthreads; ++t) order[t] = g_fiber_order.mode == 1 ? (uint32_t)(nthreads - 1 - t) : t;
+      if (g_fiber_order.mode == 2) std::shuffle(order.begin(), order.end(), rng);
+      for (uint32_t k = 0; k < nthreads; ++k) {
+        const uint32_t t = order[k];
         Fiber &f = sched.fibers[t];
         if (f.done) continue;
         any = true;

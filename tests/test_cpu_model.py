@@ -52,6 +52,21 @@ def test_cpu_model_suite(model_env, suite):
     _run_suite(model_env, suite)
 
 
+@pytest.mark.parametrize("suite", ["lde", "plans"])
+def test_cpu_model_reverse_thread_order(model_env, suite):
+    """Workgroup threads run last to first between barriers: LDS data passed
+    between threads without a barrier, or overwritten before another thread
+    has read it, would give results that differ from the oracle.  With
+    SFNTT_MODEL_FULL=1 every suite also runs in a shuffled order."""
+    _run_suite(dict(model_env, SFNTT_MODEL_ORDER="reverse"), suite)
+
+
+@pytest.mark.skipif(os.environ.get("SFNTT_MODEL_FULL") != "1", reason="full model sweep only")
+@pytest.mark.parametrize("suite", ["ntt", "lde", "plans", "noalloc", "schedules"])
+def test_cpu_model_shuffled_thread_order(model_env, suite):
+    _run_suite(dict(model_env, SFNTT_MODEL_ORDER="shuffle:7"), suite)
+
+
 @pytest.mark.parametrize("suite", ["plans", "server"])
 def test_cpu_model_few_compute_units(model_env, suite):
     """Four compute units: the many-job launches, whose grid is sized to the
