@@ -19,6 +19,8 @@
 //    lane * (T + T/32) + p + p/32 when points are.
 // The host only selects these kernels when every offset inside a workgroup
 // fits a 32-bit buffer offset.
+#include <stdlib.h>
+
 #include "pass_common.hpp"
 
 namespace sfn {
@@ -390,6 +392,68 @@ __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, L
   }
 }
 
+// ---- two-round pass with a half-tile exchange ----
+// A pass of two radix-32 rounds (LOG_T = 10) exchanges every point once
+// between its rounds.  Here the exchange goes through half a tile in two
+// phases: first the points whose top bit is 0, then those whose top bit is 1
+// (point p sits at slot p mod T/2).  Where the top point bit is one of the
+// item's own 32 points, a thread moves 16 values in each phase; where it is
+// a bit of the item's base, the thread moves all 32 in one phase.  The
+// workgroup holds half the LDS of the one-phase exchange, so three workgroups
+// of 16 lanes fit on a CU instead of two (six waves per SIMD instead of four).
+template <int LOG_T>
+__host__ __device__ constexpr size_t half_tile_words(uint32_t log_w) {
+  return (((size_t)1 << (LOG_T - 1)) + ((size_t)1 << (LOG_T - 1)) / 32) << log_w;
+}
+
+template <int LOG_T, int K, int B0, int H, class Fn>
+__device__ __forceinline__ void half_slots(uint32_t pbase, Fn &&fn) {
+  constexpr int R = 1 << K;
+  if constexpr (B0 + K == LOG_T) {
+    // top point bit = top bit of the item index: values [H * R/2, (H+1) * R/2)
+#pragma unroll
+    for (int i = H * (R / 2); i < (H + 1) * (R / 2); ++i) fn(i);
+  } else {
+    if (((pbase >> (LOG_T - 1)) & 1u) == (uint32_t)H) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) fn(i);
+    }
+  }
+}
+
+template <class F, int DIR, int LOG_T, class TW>
+__device__ __forceinline__ void half_exchange_pass(uint32_t *tile, uint32_t log_w, const LaneInfo *lanes, TW tw,
+                                                   const RoundIO &io) {
+  using S = Sched<LOG_T, KR>;
+  static_assert(S::NR == 2, "two rounds");
+  constexpr int K0 = S::template k<DIR, 0>(), B00 = S::template b0<DIR, 0>();
+  constexpr int K1 = S::template k<DIR, 1>(), B01 = S::template b0<DIR, 1>();
+  static_assert(K0 == KR && K1 == KR, "radix-32 rounds");
+  constexpr int R = 1 << KR;
+  constexpr uint32_t HM = (1u << (LOG_T - 1)) - 1;  // slot of point p: p & HM
+  // one item per thread in both rounds (blockDim == W << (LOG_T - 5)), lanes fastest
+  const uint32_t W = 1u << log_w;
+  const uint32_t lane = threadIdx.x & (W - 1);
+  const uint32_t rb = threadIdx.x >> log_w;
+  const LaneInfo &li = lanes[lane];
+  if constexpr (!F::MONTY) tw.hb = li.hb;
+  const uint32_t pb0 = round_pbase<K0, B00>(rb), pb1 = round_pbase<K1, B01>(rb);
+  uint32_t v[R], u[R];
+  load_item<F, DIR, LOG_T, true, K0, B00>(v, li, io, pb0);
+  twist_item<F, DIR, K0, B00, false>(v, li, io, pb0);
+  round_regs<F, DIR, K0, B00>(v, pb0, tw);
+  auto slot = [&](uint32_t p) { const uint32_t q = p & HM; return ((q + (q >> 5)) << log_w) + lane; };
+  half_slots<LOG_T, K0, B00, 0>(pb0, [&](int i) { tile[slot(pb0 | ((uint32_t)i << B00))] = v[i]; });
+  __syncthreads();
+  half_slots<LOG_T, K1, B01, 0>(pb1, [&](int i) { u[i] = tile[slot(pb1 | ((uint32_t)i << B01))]; });
+  __syncthreads();  // every phase-0 value is read before the slots are reused
+  half_slots<LOG_T, K0, B00, 1>(pb0, [&](int i) { tile[slot(pb0 | ((uint32_t)i << B00))] = v[i]; });
+  __syncthreads();
+  half_slots<LOG_T, K1, B01, 1>(pb1, [&](int i) { u[i] = tile[slot(pb1 | ((uint32_t)i << B01))]; });
+  round_regs<F, DIR, K1, B01>(u, pb1, tw);
+  store_item<F, DIR, LOG_T, true, K1, B01, false>(u, li, io, pb1);
+}
+
 template <class F, int DIR, int LOG_T, bool CONTIG, int LW = -1>
 __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -402,6 +466,34 @@ __global__ __launch_bounds__(512) void k_fast(const PassArgs a) {
                                                  (F::MONTY ? ((size_t)8 << LOG_T) : 0));
   if constexpr (F::MONTY) load_root<LOG_T>(a, root);
   fast_tile<F, DIR, LOG_T, CONTIG, LOG_T, LW>(a, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log), lanes, root, tile);
+}
+
+// The same pass with the half-tile exchange (CONTIG, LOG_T = 10).
+// LDS: [half tile][root table][lane records].
+template <class F, int DIR, int LOG_T, int LW = -1>
+__global__ __launch_bounds__(512) void k_fast_half(const PassArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t log_w = LW >= 0 ? (uint32_t)LW : a.log_w;  // a.log_w == LW when LW >= 0
+  uint32_t *tile = reinterpret_cast<uint32_t *>(smem);
+  uint2 *root = reinterpret_cast<uint2 *>(smem + ((half_tile_words<LOG_T>(log_w) * 4 + 15) & ~(size_t)15));
+  LaneInfo *lanes = reinterpret_cast<LaneInfo *>(reinterpret_cast<unsigned char *>(root) +
+                                                 (F::MONTY ? ((size_t)8 << LOG_T) : 0));
+  if constexpr (F::MONTY) load_root<LOG_T>(a, root);
+  RoundIO io;
+  setup<LOG_T>(a, lanes, io, xcd_tile(blockIdx.x, gridDim.x, a.xcd_share_log));
+  __syncthreads();
+  if constexpr (F::MONTY) {
+    RootTw<LOG_T, DIR> tw;
+    tw.tab = root;
+    half_exchange_pass<F, DIR, LOG_T>(tile, log_w, lanes, tw, io);
+  } else {
+    half_exchange_pass<F, DIR, LOG_T>(tile, log_w, lanes, make_tw<F, DIR, LOG_T>(a, nullptr), io);
+  }
+}
+
+__host__ __device__ constexpr size_t lds_bytes_half(int field, int log_t, uint32_t log_w) {
+  return (kLaneInfoBytes << log_w) + (field != 0 ? ((size_t)8 << log_t) : 0) +
+         (((((((size_t)1 << (log_t - 1)) + ((size_t)1 << (log_t - 1)) / 32) << log_w) * 4) + 15) & ~(size_t)15);
 }
 
 // ---- coset LDE middle pass ----
@@ -540,6 +632,23 @@ __global__ __launch_bounds__(512, LW == 4 ? 4 : 3) void k_mid_fast(const PassArg
   }
 }
 
+// SFNTT_HALF_EXCHANGE=0 keeps the one-phase exchange (A/B measurement).
+static bool half_exchange_on() {
+  static const bool on = [] {
+    const char *e = getenv("SFNTT_HALF_EXCHANGE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+static bool half_lw4_on() {
+  static const bool on = [] {
+    const char *e = getenv("SFNTT_HALF_LW4");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
 template <class F, int DIR, int LOG_T, bool CONTIG>
 static hipError_t launch_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
   const size_t lds = lds_bytes(F::ID, LOG_T, a.log_w);
@@ -560,6 +669,26 @@ static hipError_t launch_t(const PassArgs &a, uint64_t tiles, hipStream_t s) {
     return true;
   }();
   (void)configured;
+  if constexpr (CONTIG && LOG_T == 10) {
+    // lanes adjacent in memory, 2^10-point tiles: half-tile exchange
+    if (half_exchange_on()) {
+      static const bool configured_h = [&] {
+        (void)hipFuncSetAttribute((const void *)k_fast_half<F, DIR, LOG_T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+        (void)hipFuncSetAttribute((const void *)k_fast_half<F, DIR, LOG_T, 4>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+        return true;
+      }();
+      (void)configured_h;
+      const size_t lds_h = lds_bytes_half(F::ID, LOG_T, a.log_w);
+      if (a.log_w == 4 && half_lw4_on())
+        hipLaunchKernelGGL((k_fast_half<F, DIR, LOG_T, 4>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds_h, s,
+                           a);
+      else
+        hipLaunchKernelGGL((k_fast_half<F, DIR, LOG_T>), dim3((uint32_t)tiles), dim3((uint32_t)threads), lds_h, s, a);
+      return hipGetLastError();
+    }
+  }
   if (kLw4 && a.log_w == 4) hipLaunchKernelGGL(kern4, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   else hipLaunchKernelGGL(kern, dim3((uint32_t)tiles), dim3((uint32_t)threads), lds, s, a);
   return hipGetLastError();
