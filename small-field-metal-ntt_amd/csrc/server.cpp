@@ -6,12 +6,14 @@
 //    the oldest queued job it may run, plus the queued jobs it can fuse with
 //    it (continuous batching); an idle worker polls for work for kPollNs
 //    before it sleeps.  Large device-memory jobs (more than kSmallJobElems
-//    output elements) run at most kLargeDevGroups groups at a time, as they
-//    (nearly) fill the device; large host-memory jobs, bound by their PCIe
-//    copies, run two at a time,
-//    each alone in overlapped column chunks; worker 0 runs small jobs only
-//    when there are several workers, so small jobs never wait behind large
-//    ones.  Fusion:
+//    output elements) (nearly) fill the device: one runs at a time, and a
+//    second group (at most kLargeDevGroups) starts only while more large ones
+//    wait; host-memory groups, bound by their PCIe copies, run one at a time
+//    (a large one alone, in overlapped column chunks); worker 0 runs small
+//    jobs only when there are several workers, so small device-memory jobs
+//    never queue behind large ones, but while a group of kDeferElems or more
+//    runs they wait up to kSmallDeferNs (such a job beside small ones runs
+//    slower than alone, and these jobs set the tail latency).  Fusion:
 //      NTT jobs of any field, direction and size run together as one
 //      many-job launch per pass;
 //      LDE jobs of one (log_n, added_bits, shift) run as one batched LDE over
@@ -25,6 +27,7 @@
 //  - every job records submit / start / end times on the host steady clock;
 //  - destroy fails the jobs still queued with SFN_ERR_BUSY, lets the running
 //    ones finish, and returns once no thread is inside sfn_server_wait.
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -52,10 +55,19 @@ constexpr uint64_t kSmallJobElems = 1ull << 22;
 // does not fill the 256 CUs (a few hundred workgroups) leaves the rest to the
 // other, while small jobs still find free streams
 constexpr uint32_t kLargeDevGroups = 2;
-// large host-memory jobs running at once: each is bound by its PCIe copies and
-// leaves most of the device idle, and two keep both link directions busy
-// (one uploading while the other downloads)
-constexpr uint32_t kLargeHostJobs = 2;
+// a small device-memory job waits at most this long for running device-memory
+// groups of at least kDeferElems output elements (whose run of milliseconds
+// sets the tail latency) before it runs beside them
+constexpr uint64_t kSmallDeferNs = 2000000;
+constexpr uint64_t kDeferElems = 1ull << 26;
+// SFNTT_SERVER_LARGE_DEV=<n> and SFNTT_SERVER_SMALL_DEFER_US=<us> override the
+// two limits above (A/B measurement)
+uint32_t env_limit(const char *name, uint32_t dflt) {
+  const char *e = getenv(name);
+  if (!e || !*e) return dflt;
+  const long v = strtol(e, nullptr, 10);
+  return v >= 1 && v <= 32 ? (uint32_t)v : dflt;
+}
 // one fused group moves at most this many elements (bounds its latency)
 constexpr uint64_t kGroupElems = 1ull << 27;
 // an idle worker polls this long for new work before it sleeps: waking a
@@ -100,8 +112,18 @@ struct sfn_server {
   uint64_t next_ticket = 1;
   size_t reserved = 0;  // device bytes of every worker's staging buffers (held or being allocated)
   uint32_t running = 0;
-  uint32_t large_dev = 0;   // large device-memory groups running (at most kLargeDevGroups)
-  uint32_t large_host = 0;  // large host-memory jobs running (at most kLargeHostJobs)
+  uint32_t large_dev = 0;   // large device-memory groups running (at most max_large_dev)
+  uint32_t huge_dev = 0;    // device-memory groups of at least kDeferElems output elements running
+  uint32_t max_large_dev = kLargeDevGroups;
+  uint64_t small_defer_ns = kSmallDeferNs;
+  uint32_t host_groups = 0;  // host-memory groups running (at most one)
+  // host-memory groups run one at a time, so they share one staging pair and
+  // one set of transfer streams: it grows to the largest group once and is
+  // reused, where per-worker staging would be reallocated as groups move
+  // between workers (a hipFree waits for the whole device)
+  Staging host_stg;
+  size_t host_held = 0;  // host_stg's share of `reserved`
+  sfn::ChunkStreams host_cs;
   bool starved = false;        // a host-memory job waits for staging budget
   std::atomic<uint64_t> epoch{0};  // bumped by every submit and every finished group
   uint32_t waiting = 0;        // threads inside sfn_server_wait
@@ -146,23 +168,35 @@ bool sfn_server::pick(int idx, const Staging &stg, std::vector<Job *> &group, si
                       bool &staged) {
   const bool small_lane = idx == 0 && cfg.max_inflight > 1;
   // the oldest job this worker may start: small ones always; a large one when
-  // this is not the small lane and fewer than kLargeDevGroups (device memory)
-  // or kLargeHostJobs (host memory) large ones run
+  // this is not the small lane and the large-job limits allow it
+  // a second large device-memory group starts only while another large one
+  // waits behind it (a backlog): alone, a large job runs as fast as on an idle
+  // device, as in the plain path
+  uint32_t large_dev_queued = 0;
+  for (const Job *j : queue) large_dev_queued += j->req.device_ptrs && !j->small();
+  const uint64_t now = sfn_now_ns();
   auto may_start = [&](const Job *j) {
-    if (j->small()) return true;
+    // host-memory groups run one at a time: they are bound by PCIe copies,
+    // and two at once share the link, so each takes longer than alone
+    if (!j->req.device_ptrs) return host_groups == 0 && (j->small() || !small_lane);
+    // a small job beside a running large one takes CUs from it: it waits
+    // (for the large group to finish, or small_defer_ns since its submit, seen
+    // at the next submit or finished group) unless none runs
+    if (j->small()) return huge_dev == 0 || now - j->submit_ns >= small_defer_ns;
     if (small_lane) return false;
-    return j->req.device_ptrs ? large_dev < kLargeDevGroups : large_host < kLargeHostJobs;
+    return large_dev == 0 || (large_dev < max_large_dev && large_dev_queued >= 2);
   };
   auto head_it = queue.begin();
   while (head_it != queue.end() && !may_start(*head_it)) ++head_it;
   if (head_it == queue.end()) return false;
   Job *head = *head_it;
-  const size_t others = reserved - stg.held();  // the other workers' staging
+  const bool host = !head->req.device_ptrs;
+  const Staging &use = host ? host_stg : stg;
+  const size_t others = reserved - (host ? host_held : stg.held());  // every other staging
   // staging grows to at least (in, out) and never shrinks while held
   auto fits = [&](size_t in, size_t out) {
-    return others + std::max(stg.in.bytes, in) + std::max(stg.out.bytes, out) <= cfg.mem_budget_bytes;
+    return others + std::max(use.in.bytes, in) + std::max(use.out.bytes, out) <= cfg.mem_budget_bytes;
   };
-  const bool host = !head->req.device_ptrs;
   if (host && !fits(head->in_bytes, head->out_bytes)) {
     // wait for staging to free up: idle workers give theirs back
     if (!starved) cv_work.notify_all();
@@ -340,7 +374,9 @@ void sfn_server::worker_main(int idx) {
     std::vector<Job *> group;
     size_t need_in = 0, need_out = 0;
     bool staged = false, shed = false;
-    uint32_t *large = nullptr;  // the large-job counter this group holds
+    bool large = false;       // the group holds a large device-memory slot
+    bool huge = false;        // the group counts in huge_dev
+    bool host_group = false;  // the group holds the host-memory slot
     {
       std::unique_lock<std::mutex> lk(mu);
       auto ready = [&] {
@@ -362,16 +398,22 @@ void sfn_server::worker_main(int idx) {
       }
       if (group.empty() && !shed) break;  // stopping
       if (!group.empty()) {
+        host_group = !group[0]->req.device_ptrs;
         if (staged) {
-          const size_t grown = std::max(stg.in.bytes, need_in) + std::max(stg.out.bytes, need_out);
-          reserved += grown - held;
-          held = grown;
+          Staging &u = host_group ? host_stg : stg;
+          size_t &h = host_group ? host_held : held;
+          const size_t grown = std::max(u.in.bytes, need_in) + std::max(u.out.bytes, need_out);
+          reserved += grown - h;
+          h = grown;
         }
         ++running;
-        if (!group[0]->small()) {
-          large = group[0]->req.device_ptrs ? &large_dev : &large_host;
-          ++*large;
-        }
+        large = group[0]->req.device_ptrs && !group[0]->small();
+        large_dev += large;
+        uint64_t elems = 0;
+        for (const Job *j : group) elems += j->out_elems();
+        huge = large && elems >= kDeferElems;
+        huge_dev += huge;
+        host_groups += host_group;
         const uint64_t t = sfn_now_ns();
         for (Job *j : group) {
           j->start_ns = t;
@@ -385,17 +427,30 @@ void sfn_server::worker_main(int idx) {
       continue;
     }
     sfn_status st = SFN_ERR_GPU_EXECUTION;
+    Staging &u = host_group ? host_stg : stg;
+    sfn::ChunkStreams &ucs = host_group ? host_cs : cs;
     if (s) {
       st = SFN_OK;
-      if (staged && (grow(stg.in, need_in) || grow(stg.out, need_out))) st = SFN_ERR_BUFFER_ALLOC_FAILED;
-      if (!st) st = run_group(group, staged, s, stg, cs);
+      if (staged && (grow(u.in, need_in) || grow(u.out, need_out))) st = SFN_ERR_BUFFER_ALLOC_FAILED;
+      if (!st) st = run_group(group, staged, s, u, ucs);
       if (st) {
         (void)hipStreamSynchronize(s);
-        for (auto x : cs.s)
+        for (auto x : ucs.s)
           if (x) (void)hipStreamSynchronize(x);
       }
     }
-    if (st) drop_staging();  // after a failure the worker starts again from nothing
+    if (st) {  // after a failure the staging in use starts again from nothing
+      if (host_group) {
+        release(host_stg.in);
+        release(host_stg.out);
+        release(host_stg.desc);
+        std::lock_guard<std::mutex> lk(mu);
+        reserved -= host_held;
+        host_held = 0;
+      } else {
+        drop_staging();
+      }
+    }
     const uint64_t t = sfn_now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -404,7 +459,9 @@ void sfn_server::worker_main(int idx) {
         j->status = st;
         j->done = true;
       }
-      if (large) --*large;
+      large_dev -= large;
+      huge_dev -= huge;
+      host_groups -= host_group;
       --running;
     }
     epoch.fetch_add(1, std::memory_order_release);
@@ -456,6 +513,8 @@ sfn_status sfn_server_create(sfn_ctx *cx, const sfn_server_config *cfg, sfn_serv
   if (!srv->cfg.max_queue) srv->cfg.max_queue = 4096;
   if (!srv->cfg.max_group) srv->cfg.max_group = 128;
   if (!srv->cfg.max_group_cols) srv->cfg.max_group_cols = 512;
+  srv->max_large_dev = env_limit("SFNTT_SERVER_LARGE_DEV", kLargeDevGroups);
+  if (const char *e = getenv("SFNTT_SERVER_SMALL_DEFER_US")) srv->small_defer_ns = strtoull(e, nullptr, 10) * 1000;
   for (uint32_t i = 0; i < srv->cfg.max_inflight; ++i)
     srv->workers.emplace_back(&sfn_server::worker_main, srv.get(), (int)i);
   *out = srv.release();
@@ -502,7 +561,10 @@ sfn_status sfn_server_submit(sfn_server *srv, const sfn_request *req, uint64_t *
   srv->queue.push_back(job.get());
   srv->jobs.emplace(job->ticket, std::move(job));
   srv->epoch.fetch_add(1, std::memory_order_release);
-  srv->cv_work.notify_one();
+  // every sleeping worker rechecks: one woken alone may be unable to run the
+  // job (the small-job lane, a busy host-memory slot) and sleep again, and
+  // the job would then wait for the next submit or finished group
+  srv->cv_work.notify_all();
   return SFN_OK;
 }
 
@@ -571,6 +633,14 @@ void sfn_server_destroy(sfn_server *srv) {
     std::unique_lock<std::mutex> lk(srv->mu);
     srv->cv_done.wait(lk, [&] { return srv->waiting == 0; });
   }
+  // the workers are gone: the host-memory groups' shared staging and streams
+  (void)hipSetDevice(srv->ctx->device);
+  for (auto x : srv->host_cs.s)
+    if (x) (void)hipStreamSynchronize(x);
+  release(srv->host_stg.in);
+  release(srv->host_stg.out);
+  release(srv->host_stg.desc);
+  srv->host_cs.release();
   delete srv;
 }
 

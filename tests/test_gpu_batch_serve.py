@@ -421,3 +421,36 @@ def test_server_fuses_jobs_of_every_ntt_shape_and_same_shape_ldes(ctx):
         assert np.array_equal(got, exp), (kind, f, log_n, batch, d, dev)
     ctx._lib.sfn_server_destroy(srv)
     assert max(groups) > 1
+
+
+def test_lone_large_job_starts_while_workers_sleep(ctx):
+    """A large job submitted while every worker sleeps starts at once.  The
+    small-job lane cannot run it, so a wake-up that reached only that worker
+    would leave the job queued until the next submit (regression test)."""
+    log_n = 23  # 2^23 output elements: a large job
+    x = synth.field_data(0, 77, 1 << log_n)
+    din, dout = sfntt.DeviceBuffer(ctx, x.size), sfntt.DeviceBuffer(ctx, x.size)
+    din.upload(x)
+    lib = ctx._lib
+    srv = make_server(ctx, inflight=4)
+    try:
+        for rep in range(12):
+            time.sleep(0.01)  # idle workers poll for 100 us, then sleep
+            r = L.Request(L.JOB_NTT, 0, L.FORWARD, log_n, 1, 0, 0, 1, din.ptr, dout.ptr)
+            t = ctypes.c_uint64()
+            ctx.check(lib.sfn_server_submit(srv, ctypes.byref(r), ctypes.byref(t)))
+            tm = L.JobTiming()
+            deadline = time.time() + 5.0
+            while True:
+                st = lib.sfn_server_poll(srv, t.value, ctypes.byref(tm))
+                if st != L.ERR_NOT_READY:
+                    break
+                assert time.time() < deadline, f"repeat {rep}: the job never started"
+                time.sleep(0.0005)
+            ctx.check(st)
+            assert tm.start_ns - tm.submit_ns < 50_000_000, f"repeat {rep}: started after {tm.start_ns - tm.submit_ns} ns"
+    finally:
+        lib.sfn_server_destroy(srv)
+    assert np.array_equal(dout.download(), orc.m31_forward(x))
+    din.free()
+    dout.free()
