@@ -631,10 +631,14 @@ bool choose_lanes_fast(PassDesc &p, int layout, uint32_t batch, uint32_t want_lo
 }
 
 // Tuned passes whose lanes are adjacent elements: a tile row spans 4 * W
-// bytes, so 128 / (4 * W) consecutive tiles share every 128-byte line.
+// bytes, so 128 / (4 * W) consecutive tiles share every 128-byte line; eight
+// consecutive tiles go to one XCD (with 16 lanes, 512 contiguous bytes of
+// every row).  Measured on the 2^20 x 256 LDE against the 128 / (4 * W)
+// tiles that share a line: outer passes 1071 -> 968 us (forward) and
+// 525 -> 506 us (inverse); four tiles were slower than either.
 uint32_t xcd_share_log(const Context *ctx, const PassDesc &p) {
   if (!ctx->xcd_remap || !p.lane_contig || p.log_w >= 5) return 0;
-  return std::min<uint32_t>(3, 5 - p.log_w);
+  return 3;
 }
 
 }  // namespace
