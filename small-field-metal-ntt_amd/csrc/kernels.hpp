@@ -25,6 +25,9 @@ struct PassArgs {
   uint64_t src_cs, src_rs;
   uint64_t dst_cs, dst_rs;
   uint64_t instances;   // batch << (log_n - LOG_T)
+  // ceil(2^64 / batch) when batch >= 2 and instances <= 2^32 (lane decoding
+  // divides by batch with two multiplies), else 0 (a 64-bit division)
+  uint64_t batch_recip;
   uint32_t log_n;
   uint32_t lo;
   uint32_t batch;

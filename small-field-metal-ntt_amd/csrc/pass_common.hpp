@@ -37,6 +37,30 @@ struct LaneInfo {
   uint32_t low, valid;
 };
 
+// floor(n / d) for n < 2^32 from m = ceil(2^64 / d), d >= 2: (m * n) >> 64, exact
+// for every such n (Lemire, Kaser and Kurz, "Faster remainder by direct
+// computation", 2019, theorem 1 with F = 64, N = 32).  m * n = m_hi * n * 2^32 +
+// m_lo * n, and the low 32 bits of m_lo * n cannot carry into bit 64.
+__device__ __forceinline__ uint32_t div_by_recip(uint32_t n, uint64_t m) {
+  const uint64_t t = (uint64_t)(uint32_t)(m >> 32) * n + __umulhi((uint32_t)m, n);
+  return (uint32_t)(t >> 32);
+}
+
+// x / batch and x % batch
+__device__ __forceinline__ void div_batch(const PassArgs &a, uint64_t x, uint64_t &q, uint32_t &r) {
+  if (a.batch == 1) {
+    q = x;
+    r = 0;
+  } else if (a.batch_recip) {  // x < 2^32 (the host checked instances <= 2^32)
+    const uint32_t qq = div_by_recip((uint32_t)x, a.batch_recip);
+    q = qq;
+    r = (uint32_t)x - qq * a.batch;
+  } else {
+    q = x / a.batch;
+    r = (uint32_t)(x % a.batch);
+  }
+}
+
 __device__ __forceinline__ void decode_lane(const PassArgs &a, int log_t, uint64_t g, LaneInfo &li) {
   const uint32_t lo = a.lo;
   const uint32_t hi = lo + log_t;
@@ -47,12 +71,10 @@ __device__ __forceinline__ void decode_lane(const PassArgs &a, int log_t, uint64
   if (!li.valid) g = 0;
   if (a.order == kLowFastest) {
     low = (uint32_t)g & lo_mask;
-    uint64_t r = g >> lo;
-    c = (uint32_t)(r % a.batch);
-    hb = r / a.batch;
+    div_batch(a, g >> lo, hb, c);
   } else if (a.order == kColFastest) {
-    c = (uint32_t)(g % a.batch);
-    uint64_t r = g / a.batch;
+    uint64_t r;
+    div_batch(a, g, r, c);
     low = (uint32_t)r & lo_mask;
     hb = r >> lo;
   } else {

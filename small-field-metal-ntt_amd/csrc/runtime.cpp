@@ -698,6 +698,13 @@ static const char *field_tag(int field) {
   return field == SFN_FIELD_M31 ? "m31," : (field == SFN_FIELD_KOALABEAR ? "kb," : "bb,");
 }
 
+// ceil(2^64 / batch) for the kernels' lane decoding (PassArgs::batch_recip), or 0
+// where it does not apply (one column, or lane indices past 32 bits)
+uint64_t batch_reciprocal(uint32_t batch, uint64_t instances) {
+  if (batch < 2 || instances > (1ull << 32)) return 0;
+  return ~0ull / batch + 1;
+}
+
 sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, const uint32_t *src, uint32_t *dst,
                           uint64_t src_cs, uint64_t src_rs, uint64_t dst_cs, uint64_t dst_rs, uint32_t batch,
                           uint32_t log_n_real, PassArgs *out) {
@@ -716,6 +723,7 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   a.order = p.order;
   a.lane_contig = p.lane_contig;
   a.instances = (uint64_t)batch << (p.log_n - p.log_t);
+  a.batch_recip = batch_reciprocal(batch, a.instances);
   a.src_rev_bits = p.src_rev;
   a.dst_rev_bits = p.dst_rev;
   if (field != SFN_FIELD_M31) {
@@ -1007,6 +1015,7 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     a.order = p.order;
     a.lane_contig = p.lane_contig;
     a.instances = (uint64_t)batch << lo;
+    a.batch_recip = batch_reciprocal(batch, a.instances);
     const uint2 *pair, *twi, *twf, *mid;
     if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, true, &twi))) return st;
     if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, false, &twf))) return st;

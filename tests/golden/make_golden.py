@@ -11,7 +11,9 @@ digests.json   SHA-256 of the full outputs of the benchmark workloads, over
                  m31_fwd_2e20    M31 forward 2^20, seed 12345 + 2^20
                  lde_2e20x256_*  coset LDE 2^20 x 256, 2x, seed 42 + 20 (inputs
                                  generated column by column): row-major with
-                                 shift 31, and SoA with the reference shift
+                                 shift 31, and SoA with the reference shift;
+                                 _natural: row-major, shift 31, natural row
+                                 order on both sides
                  uneven_512_columns_forward
                                  512 uneven columns (SplitMix64 seed 0x5EED0004:
                                  field = draw1 & 1, log_n = 10 + draw2 % 13, LCG
@@ -39,6 +41,15 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import oracle as orc  # noqa: E402
 
 M31, BB = orc.FIELD_M31, orc.FIELD_BB
+
+
+def bitrev_index(bits: int) -> np.ndarray:
+    """i -> i with its low `bits` bits reversed, for i < 2^bits"""
+    i = np.arange(1 << bits, dtype=np.uint64)
+    r = np.zeros_like(i)
+    for b in range(bits):
+        r |= ((i >> np.uint64(b)) & np.uint64(1)) << np.uint64(bits - 1 - b)
+    return r.astype(np.int64)
 
 
 def sha(a: np.ndarray) -> str:
@@ -112,6 +123,13 @@ def digests() -> dict:
     d["lde_2e20x256_2x_soa_refshift"] = sha(y)
     y = orc.coset_lde_batch(x, log_n, B, a, orc.bb_to_monty(31))  # SoA result, shift 31
     d["lde_2e20x256_2x_rowmajor_shift31"] = sha(y.reshape(B, 2 << log_n).T)
+    # natural row order on both sides (input row i = P(g_n^i), output row k =
+    # P(s g_N^k)): the same columns read as natural-order rows; the natural LDE
+    # is the bit-reversed-order LDE between two row bit-reversals
+    rn, rN = bitrev_index(log_n), bitrev_index(log_n + a)
+    yn = orc.coset_lde_batch(np.ascontiguousarray(x.reshape(B, 1 << log_n)[:, rn]).ravel(), log_n, B, a,
+                             orc.bb_to_monty(31)).reshape(B, 2 << log_n)[:, rN]
+    d["lde_2e20x256_2x_rowmajor_shift31_natural"] = sha(yn.T)
     h = hashlib.sha256()
     for field, ln, seed in uneven_shapes():
         x = orc.lcg_fill(seed, 1 << ln, field)

@@ -95,6 +95,12 @@ def test_lde_2e20x256_digests(ctx, digests):
                                                       d_out.ptr, None))
         ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
         assert sha(d_out.download()) == digests["lde_2e20x256_2x_soa_refshift"]
+        print("[golden] natural-order lde 2^20 x 256 row-major shift 31", flush=True)
+        d_in.upload(x.reshape(B, n).T.copy().ravel())  # the same columns as natural-order rows
+        ctx.check(ctx._lib.sfn_coset_lde_natural_device(ctx.ptr, log_n, B, a, orc.bb_to_monty(31),
+                                                        L.LAYOUT_ROW_MAJOR, d_in.ptr, d_out.ptr, None))
+        ctx.check(ctx._lib.sfn_synchronize(ctx.ptr, None))
+        assert sha(d_out.download()) == digests["lde_2e20x256_2x_rowmajor_shift31_natural"]
     finally:
         d_in.free()
         d_out.free()
