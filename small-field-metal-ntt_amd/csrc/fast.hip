@@ -21,6 +21,8 @@
 // fits a 32-bit buffer offset.
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "pass_common.hpp"
 
 namespace sfn {
@@ -374,7 +376,8 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
 // offset from one per-item base.
 // KRT: layers per register round (5; 6 gives a 2^11-point pass two rounds,
 // 6 + 5, instead of 5 + 5 + 1, for 64 values per item in the 6-layer round).
-template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG, int LW = -1, int KRT = KR>
+// HALF: the root table holds only e <= 2^(ROOT_LOG-1) (HalfRootTw).
+template <class F, int DIR, int LOG_T, bool CONTIG, int ROOT_LOG, int LW = -1, int KRT = KR, bool HALF = false>
 __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, LaneInfo *lanes, const uint2 *root,
                                           uint32_t *tile) {
   static_assert(ROOT_LOG >= LOG_T, "root table too small");
@@ -383,7 +386,7 @@ __device__ __forceinline__ void fast_tile(const PassArgs &a, uint32_t tile_id, L
   setup<LOG_T>(a, lanes, io, tile_id);
   __syncthreads();
   if constexpr (F::MONTY) {
-    RootTw<ROOT_LOG, DIR> tw;
+    std::conditional_t<HALF, HalfRootTw<ROOT_LOG, DIR>, RootTw<ROOT_LOG, DIR>> tw;
     tw.tab = root;
     rounds<F, DIR, LOG_T, CONTIG, 0, Sched<LOG_T, KRT>::NR, false, KRT>(tile, log_w, lanes, tw, io);
   } else {
@@ -641,10 +644,13 @@ static bool half_exchange_on() {
   return on;
 }
 
+// 16 lanes fixed at compile time in the half-tile exchange kernels (measured:
+// LDE inverse pass 502-515 -> 480 us, forward unchanged); SFNTT_HALF_LW4=0 keeps
+// the runtime lane count (A/B measurement)
 static bool half_lw4_on() {
   static const bool on = [] {
     const char *e = getenv("SFNTT_HALF_LW4");
-    return e && e[0] == '1';
+    return !(e && e[0] == '0');
   }();
   return on;
 }
@@ -816,28 +822,30 @@ __device__ __forceinline__ void multi_tile(uint32_t log_t, const PassArgs &a, ui
                                            const uint2 *root, uint32_t *tile_lds) {
   switch (log_t) {
 #define SFN_CASE(L) \
-  case L: fast_tile<F, DIR, L, CONTIG, kMaxLogTile>(a, tile, lanes, root, tile_lds); break;
-    SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10)
+  case L: fast_tile<F, DIR, L, CONTIG, kMaxLogTile, -1, KR, true>(a, tile, lanes, root, tile_lds); break;
+    SFN_CASE(6) SFN_CASE(7) SFN_CASE(8) SFN_CASE(9) SFN_CASE(10) SFN_CASE(11)
 #undef SFN_CASE
-    // 2^11 points in two rounds (6 + 5 layers), so the largest columns of a
-    // launch set take two passes of 11 layers
-    case 11: fast_tile<F, DIR, 11, CONTIG, kMaxLogTile, -1, 6>(a, tile, lanes, root, tile_lds); break;
     default: break;
   }
 }
 
-// LDS: [root table of the launch's Montgomery field (BabyBear or KoalaBear;
-// one per launch), 2^11 Shoup pairs, loaded once][lane records: lanes_bytes][tile]
-__global__ __launch_bounds__(kMultiFastThreads) void k_multi_fast(const SegDesc *segs, uint32_t nseg,
+// LDS: [half root table of the launch's Montgomery field (BabyBear or
+// KoalaBear; one per launch): g^e for e <= 2^10 of the 2^11-entry table, loaded
+// once][lane records: lanes_bytes][tile].  Every tile runs radix-32 rounds (a
+// 2^11-point tile three, 5 + 5 + 1 layers), which keeps the kernel at 128
+// VGPRs: 512 threads per workgroup and two workgroups per CU, four waves per
+// SIMD (measured on 512 uneven columns: 2.64 -> 2.44 ms against 256 threads with
+// a 6 + 5 round schedule for 2^11, 144 VGPRs and three waves per SIMD).
+__global__ __launch_bounds__(kMultiFastThreads, 4) void k_multi_fast(const SegDesc *segs, uint32_t nseg,
                                                                   uint64_t total_tiles, uint32_t xs,
                                                                   const uint2 *root_g, uint32_t lanes_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint2 *root = reinterpret_cast<uint2 *>(smem);
-  const size_t root_bytes = root_g ? ((size_t)8 << kMaxLogTile) : 0;
+  const size_t root_bytes = root_g ? kMultiRootBytes : 0;
   LaneInfo *lanes = reinterpret_cast<LaneInfo *>(smem + root_bytes);
   uint32_t *tile_lds = reinterpret_cast<uint32_t *>(smem + root_bytes + lanes_bytes);
   if (root_g)
-    for (uint32_t e = threadIdx.x; e < (1u << kMaxLogTile); e += blockDim.x) root[e] = root_g[e];
+    for (uint32_t e = threadIdx.x; e <= (1u << (kMaxLogTile - 1)); e += blockDim.x) root[e] = root_g[e];
   const uint64_t gs = 8ull << xs;
   const uint64_t full = total_tiles - total_tiles % gs;
   for (uint64_t t0 = blockIdx.x; t0 < total_tiles; t0 += gridDim.x) {

@@ -107,10 +107,14 @@ hipError_t launch_fast(int field, int dir, int log_t, bool contig, const PassArg
 hipError_t launch_lde_mid_fast(int log_t, bool contig, const PassArgs &a, uint64_t tiles, hipStream_t s);
 // Tuned multi-job pass: segments' kind = field * 2 + dir (+ 4 when lanes are
 // adjacent in memory), log_t in [6, 11]; kMultiFastThreads threads per
-// workgroup.  bb_root: the BabyBear root table (2^11 Shoup pairs) when any
-// segment is BabyBear, else null; lanes_bytes: the lane-record region.
-// LDS = (bb_root ? 16 KiB : 0) + lanes_bytes + the largest tile.
-constexpr int kMultiFastThreads = 256;
+// workgroup (at most 128 VGPRs: two workgroups, four waves per SIMD).
+// bb_root: the root table of the launch's Montgomery field (2^11 Shoup pairs,
+// of which the kernel stages the half e <= 2^10) when any segment has one, else
+// null; lanes_bytes: the lane-record region.
+// LDS = (bb_root ? kMultiRootBytes : 0) + lanes_bytes + the largest tile.
+constexpr int kMultiFastThreads = 512;
+// half root table of the many-job kernel: g^e for e <= 2^10, 16-byte padded
+constexpr size_t kMultiRootBytes = ((((size_t)1 << 10) + 1) * 8 + 15) & ~(size_t)15;
 hipError_t launch_multi_fast(const SegDesc *d_segs, uint32_t nseg, uint64_t total_tiles, size_t lds, uint32_t grid,
                              uint32_t xs, const void *bb_root, uint32_t lanes_bytes, hipStream_t s);
 hipError_t launch_copy(const uint32_t *src, uint32_t *dst, uint64_t len, hipStream_t s);

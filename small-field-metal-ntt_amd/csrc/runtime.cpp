@@ -1101,7 +1101,7 @@ namespace {
 // 32-point item per thread of a 256-thread workgroup.
 bool choose_lanes_multi(PassDesc &p, uint32_t batch) {
   if (p.log_t < 6 || p.log_t > (int)kMaxLogTile || p.logr != 5) return false;
-  uint32_t lw = std::min<uint32_t>(6, 13 - (uint32_t)p.log_t);  // lane records <= 2 KiB
+  uint32_t lw = std::min<uint32_t>(7, 14 - (uint32_t)p.log_t);  // lane records <= 4 KiB
   const uint64_t instances = (uint64_t)batch << (p.log_n - p.log_t);
   while (lw > 0 && (1ull << (lw - 1)) >= instances) --lw;  // no wholly idle lane groups
   if (p.lo > 0) {
@@ -1112,7 +1112,8 @@ bool choose_lanes_multi(PassDesc &p, uint32_t batch) {
     p.order = kHighFastest;  // lanes: contiguous sub-transforms (points adjacent)
     p.lane_contig = 0;
   }
-  if (fast_lds_bytes(1, p.log_t, lw) > kMaxLdsBytes / 2) return false;
+  const size_t T = (size_t)1 << p.log_t;
+  if (kMultiRootBytes + (kLaneInfoBytes << lw) + ((T + T / 32) << lw) * 4 > kMaxLdsBytes / 2) return false;
   p.log_w = lw;
   p.fast = true;
   p.tiles = (instances + (1ull << lw) - 1) >> lw;
@@ -1147,9 +1148,10 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
     Schedule sc = ctx->schedule_for(m.field, m.log_n, m.batch, SFN_LAYOUT_SOA);
     // Many jobs fill the device together, so passes may be large: up to 2^11
     // points, except that two 11-layer passes lose to three shorter ones on
-    // 2^22 columns (measured: the 2^11-point tile holds only 4 lanes, so its
-    // strided pass moves 16-byte row pieces).  A tuned entry keeps its pass
-    // size; the radix is always the tuned kernel's.
+    // 2^22 columns (measured on 512 uneven columns: 2.50 vs 2.44 ms; the
+    // 2^11-point tile holds 8 lanes, so its strided pass moves 32-byte row
+    // pieces).  A tuned entry keeps its pass size; the radix is always the
+    // tuned kernel's.
     if (!ctx->has_tuning(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}))
       sc.max_log_t = m.log_n > 2 * (kMaxLogTile - 1) + 1 ? kMaxLogTile - 1 : kMaxLogTile;
     sc.logr = 5;
@@ -1209,7 +1211,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
       L.tiles += p.tiles;
       L.segs.push_back(d);
     }
-    if (tuned) L.lds = (L.mf_root ? ((size_t)8 << kMaxLogTile) : 0) + L.lanes_bytes + tile_bytes;
+    if (tuned) L.lds = (L.mf_root ? kMultiRootBytes : 0) + L.lanes_bytes + tile_bytes;
   }
   return SFN_OK;
 }
@@ -1218,7 +1220,7 @@ uint32_t multi_grid(const Context *ctx, const MultiLaunch &L) {
   const uint64_t cus = (uint64_t)std::max(1, ctx->props.multiProcessorCount);
   if (L.tuned) {
     // resident workgroups per CU by LDS (at most 4 of 4 waves), a multiple of 8
-    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(4, kMaxLdsBytes / std::max<size_t>(L.lds, 1)));
+    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(2, kMaxLdsBytes / std::max<size_t>(L.lds, 1)));
     const uint64_t cap = std::max<uint64_t>(8, (cus * per_cu) & ~7ull);
     const uint64_t want = (L.tiles + 7) & ~7ull;
     return (uint32_t)std::min(cap, want);
