@@ -82,11 +82,19 @@ __device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *src, 
 template <int LOG_T, int DIR>
 struct RootTw {
   static constexpr bool kNeg = false;
+  static constexpr bool kFoldRoots = true;
   const uint2 *tab;
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
     uint32_t e = pos << (LOG_T - t - 1);
     if (DIR == 1) e = (0u - e) & ((1u << LOG_T) - 1);
     return tab[e];
+  }
+  // the same entry of the compile-time table (t, pos constants)
+  template <class F>
+  __device__ static __forceinline__ uint2 cget(int t, uint32_t pos) {
+    uint32_t e = pos << (LOG_T - t - 1);
+    if (DIR == 1) e = (0u - e) & ((1u << LOG_T) - 1);
+    return make_uint2(kConstRoots<F, LOG_T>.w[e], kConstRoots<F, LOG_T>.q[e]);
   }
 };
 
@@ -97,10 +105,17 @@ struct RootTw {
 template <int LOG_T, int DIR>
 struct HalfRootTw {
   static constexpr bool kNeg = DIR == 1;
+  static constexpr bool kFoldRoots = true;
   const uint2 *tab;
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
     const uint32_t e = pos << (LOG_T - t - 1);
     return tab[DIR == 1 ? (1u << (LOG_T - 1)) - e : e];
+  }
+  template <class F>
+  __device__ static __forceinline__ uint2 cget(int t, uint32_t pos) {
+    const uint32_t e = pos << (LOG_T - t - 1);
+    const uint32_t i = DIR == 1 ? (1u << (LOG_T - 1)) - e : e;
+    return make_uint2(kConstRoots<F, LOG_T>.w[i], kConstRoots<F, LOG_T>.q[i]);
   }
 };
 

@@ -46,12 +46,16 @@ struct M31 {
 
 // PINV = p^-1 mod 2^32; both moduli are below 2^31, so sums of two canonical
 // values and the Shoup remainders fit in 32 bits.
-template <uint32_t P_, uint32_t PINV_, int ID_>
+template <uint32_t P_, uint32_t PINV_, int ID_, uint32_t ROOT_, int LOG_ROOT_>
 struct MontyField {
   static constexpr uint32_t P = P_;
   static constexpr uint32_t PINV = PINV_;
   static constexpr uint32_t PINV_NEG = 0u - PINV_;  // -p^-1 mod 2^32
   static constexpr int ID = ID_;
+  // a primitive 2^LOG_ROOT-th root of unity (standard form): the field's
+  // two-adic generator, from which every root table is derived
+  static constexpr uint32_t ROOT = ROOT_;
+  static constexpr int LOG_ROOT = LOG_ROOT_;
   static constexpr bool MONTY = true;
   using Tw = uint2;  // .x = standard value w, .y = floor(w * 2^32 / p)
   static_assert((uint32_t)(P_ * PINV_) == 1u, "PINV is p^-1 mod 2^32");
@@ -88,8 +92,31 @@ struct MontyField {
   }
 };
 
-using BabyBear = MontyField<0x78000001u, 0x88000001u, 1>;
-using KoalaBear = MontyField<0x7F000001u, 0x81000001u, 2>;
+// BabyBear's 2^27-th root: src/field/babybear.rs:31-33; KoalaBear's 2^24-th
+// root: 3^127 (3 generates the multiplicative group, p - 1 = 2^24 * 127)
+using BabyBear = MontyField<0x78000001u, 0x88000001u, 1, 440564289u, 27>;
+using KoalaBear = MontyField<0x7F000001u, 0x81000001u, 2, 1791270792u, 24>;
+
+// Compile-time root table g^e, e < 2^LOG, g = a primitive 2^LOG-th root of F,
+// as Shoup pairs (w, floor(w * 2^32 / p)): the values the host's root tables
+// hold.  Indexed with constants (after unrolling), its entries fold into
+// instruction operands and occupy no memory.
+template <class F, int LOG>
+struct ConstRoots {
+  uint32_t w[1 << LOG], q[1 << LOG];
+  constexpr ConstRoots() : w{}, q{} {
+    uint64_t g = F::ROOT;
+    for (int k = LOG; k < F::LOG_ROOT; ++k) g = g * g % F::P;
+    uint64_t x = 1;
+    for (int e = 0; e < (1 << LOG); ++e) {
+      w[e] = (uint32_t)x;
+      q[e] = (uint32_t)((x << 32) / F::P);
+      x = x * g % F::P;
+    }
+  }
+};
+template <class F, int LOG>
+__device__ constexpr ConstRoots<F, LOG> kConstRoots{};
 
 // ---- butterflies ----
 // CT (decimation in time):  (x, y) -> (x + w*y, x - w*y)

@@ -103,6 +103,7 @@ struct TwCtx;
 template <class F>
 struct TwCtx<F, true> {
   static constexpr bool kNeg = false;
+  static constexpr bool kFoldRoots = false;
   const uint2 *stw;  // LDS
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const { return stw[(1u << t) - 1 + pos]; }
 };
@@ -110,6 +111,7 @@ struct TwCtx<F, true> {
 template <>
 struct TwCtx<M31, false> {
   static constexpr bool kNeg = false;
+  static constexpr bool kFoldRoots = false;
   const uint32_t *tbl;  // global, whole transform
   uint32_t n_log, lo, hi;
   uint64_t hb;
@@ -135,7 +137,9 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
       for (int m = 0; m < (1 << (F::MONTY ? j : (K - 1 - j))); ++m) {
         typename F::Tw w;
         if constexpr (F::MONTY) {
-          w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
+          // a round over the lowest point bits: every position is a constant
+          if constexpr (B0 == 0 && TW::kFoldRoots) w = TW::template cget<F>(j, (uint32_t)m);
+          else w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
           w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
         }
@@ -162,7 +166,8 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
       for (int m = 0; m < (1 << (F::MONTY ? j : (K - 1 - j))); ++m) {
         typename F::Tw w;
         if constexpr (F::MONTY) {
-          w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
+          if constexpr (B0 == 0 && TW::kFoldRoots) w = TW::template cget<F>(j, (uint32_t)m);
+          else w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
           w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
         }
