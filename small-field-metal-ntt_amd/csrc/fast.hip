@@ -13,7 +13,9 @@
 //  - keep one root table g_T^e (e < T) in LDS that serves every layer and both
 //    directions (layer t, position j uses e = j * 2^(LOG_T-t-1), negated for
 //    the inverse);
-//  - stream the four-step twists as single-word Montgomery constants;
+//  - stream the four-step twists as single-word Montgomery constants, or,
+//    where the lanes of a workgroup do not share their twist, generate them
+//    (two root powers and a geometric sequence per item);
 //  - lay LDS out so the exchanges are bank-conflict free: lane-fastest
 //    (p + p/32) * W + lane when lanes are adjacent in memory, lane-major
 //    lane * (T + T/32) + p + p/32 when points are.
@@ -137,7 +139,7 @@ __device__ __forceinline__ uint32_t lidx(uint32_t lane, uint32_t p, uint32_t log
 // Per-workgroup state shared by the rounds.  All offsets are relative to the
 // workgroup's first lane (lane 0 has the lowest address of the tile).
 struct RoundIO {
-  Rsrc src, dst, twist;
+  Rsrc src, dst, twist, pow;
   const uint32_t *srcp;  // plain pointers (point-adjacent variant), already offset to lane 0
   uint32_t *dstp;
   uint64_t sbase0, dbase0;  // element index of lane 0's row 0 in src / dst
@@ -152,6 +154,7 @@ struct RoundIO {
   // (LDE middle pass with column-fastest lanes, which all share low0)
   const uint32_t *ltw, *lpost;
   bool has_twist;
+  bool twist_gen;  // has_twist: generate the twist from `pow` (else read `twist`)
   uint32_t has_scale, scale_w, scale_q;
   // LDE blocks: per-point factor table (one block) and post-twist table (one block)
   Rsrc pointf, post;
@@ -199,14 +202,50 @@ __device__ __forceinline__ void load_item(uint32_t (&v)[1 << K], const LaneInfo 
   }
 }
 
+// g_hi^(+-e) in Montgomery form, e < 2^hi, for the pass's four-step root
+// g_hi = g^(2^(LOG_ROOT - hi)), hi = lo + LOG_T: g^E with E = e * 2^(LOG_ROOT - hi)
+// (negated mod 2^LOG_ROOT for the inverse) is the product of two table words.
+template <class F, int DIR, int LOG_T>
+__device__ __forceinline__ uint32_t twist_root_pow(const RoundIO &io, uint32_t e) {
+  constexpr uint32_t kMask = (1u << F::LOG_ROOT) - 1;
+  uint32_t E = e << (F::LOG_ROOT - LOG_T - io.lo);
+  if (DIR == 1) E = (0u - E) & kMask;
+  return F::mul(bload(io.pow, (E & ((1u << kPowLoBits) - 1)) * 4u, 0),
+                bload(io.pow, (E >> kPowLoBits) * 4u, 4u << kPowLoBits));
+}
+
+// Multiplies the item's values by their four-step twist, generated: the twist
+// of (point p, low row bits l) is g_hi^(+-l * bitrev(p, LOG_T)), and for the
+// item's points p = pbase | i << B0 (pbase zero on bits [B0, B0 + K)),
+// bitrev(p) = bitrev(pbase) + bitrev_K(i) << (LOG_T - B0 - K): so the twist of
+// point i is s * r^bitrev_K(i) with s = g_hi^(l * bitrev(pbase)) and
+// r = g_hi^(l << (LOG_T - B0 - K)), a geometric sequence walked with one
+// Montgomery product per point.  The walk stays in [0, 2p) (no final
+// reduction: s < 2p, r < p keep s * r < p * 2^32), and the product with the
+// value (< p) is reduced to [0, p).
+template <class F, int DIR, int LOG_T, int K, int B0>
+__device__ __forceinline__ void twist_gen(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
+                                          uint32_t pbase) {
+  constexpr int R = 1 << K;
+  const uint32_t brp = __brev(pbase) >> (32 - LOG_T);
+  uint32_t s = twist_root_pow<F, DIR, LOG_T>(io, li.low * brp);
+  const uint32_t r = twist_root_pow<F, DIR, LOG_T>(io, li.low << (LOG_T - B0 - K));
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    v[brev_c<K>(k)] = F::mul(v[brev_c<K>(k)], s);
+    if (k + 1 < R) s = F::mul_lazy(s, r);
+  }
+}
+
 // Four-step twist of an inverse Montgomery-field pass, applied to loaded values.
-template <class F, int DIR, int K, int B0, bool LT>
+template <class F, int DIR, int LOG_T, int K, int B0, bool LT>
 __device__ __forceinline__ void twist_item(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
                                            uint32_t pbase) {
   constexpr int R = 1 << K;
   if constexpr (F::MONTY && DIR == 1) {
     if (io.has_twist && li.valid) {
       if constexpr (LT) mul_lds_table<F, R, B0>(v, io.ltw, pbase);
+      else if (io.twist_gen) twist_gen<F, DIR, LOG_T, K, B0>(v, li, io, pbase);
       else mul_table<F, R>(v, io.twist, (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u, 4u << (B0 + io.tps));
     }
   }
@@ -221,7 +260,10 @@ __device__ __forceinline__ void store_item(uint32_t (&v)[1 << K], const LaneInfo
   if (!li.valid) return;
   if constexpr (F::MONTY && DIR == 0) {
     const uint32_t toff = (((uint32_t)pbase << io.tps) + (li.low << io.tls)) * 4u;
-    if (io.has_twist) mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
+    if (io.has_twist) {
+      if (io.twist_gen) twist_gen<F, DIR, LOG_T, K, B0>(v, li, io, pbase);
+      else mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
+    }
     if (io.has_post) {
       if constexpr (LT) {
         // keep the 32 table reads after the butterflies: hoisting them
@@ -281,7 +323,7 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
     uint32_t v[R];
     if constexpr (FIRST) {
       load_item<F, DIR, LOG_T, CONTIG, K, B0>(v, li, io, pbase);
-      twist_item<F, DIR, K, B0, LT>(v, li, io, pbase);
+      twist_item<F, DIR, LOG_T, K, B0, LT>(v, li, io, pbase);
     } else {
 #pragma unroll
       for (int i = 0; i < R; ++i) v[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
@@ -372,8 +414,10 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.ltw = io.lpost = nullptr;
   io.tps = a.tab_low_major ? 0u : a.lo;
   io.tls = a.tab_low_major ? (uint32_t)LOG_T : 0u;
-  io.has_twist = a.twist_m != nullptr;
-  if (io.has_twist) io.twist = make_rsrc(a.twist_m, (uint64_t)4 << (a.lo + LOG_T));
+  io.twist_gen = a.pow_m != nullptr;
+  io.has_twist = a.twist_m != nullptr || io.twist_gen;
+  if (a.twist_m) io.twist = make_rsrc(a.twist_m, (uint64_t)4 << (a.lo + LOG_T));
+  if (io.twist_gen) io.pow = make_rsrc(a.pow_m, kPowWords * 4);
   io.has_scale = a.has_scale;
   io.scale_w = a.scale_w;
   io.scale_q = a.scale_q;
@@ -458,7 +502,7 @@ __device__ __forceinline__ void half_exchange_pass(uint32_t *tile, uint32_t log_
   const uint32_t pb0 = round_pbase<K0, B00>(rb), pb1 = round_pbase<K1, B01>(rb);
   uint32_t v[R], u[R];
   load_item<F, DIR, LOG_T, true, K0, B00>(v, li, io, pb0);
-  twist_item<F, DIR, K0, B00, false>(v, li, io, pb0);
+  twist_item<F, DIR, LOG_T, K0, B00, false>(v, li, io, pb0);
   round_regs<F, DIR, K0, B00>(v, pb0, tw);
   auto slot = [&](uint32_t p) { const uint32_t q = p & HM; return ((q + (q >> 5)) << log_w) + lane; };
   half_slots<LOG_T, K0, B00, 0>(pb0, [&](int i) { tile[slot(pb0 | ((uint32_t)i << B00))] = v[i]; });

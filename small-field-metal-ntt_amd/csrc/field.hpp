@@ -83,7 +83,15 @@ struct MontyField {
     const uint32_t r = (uint32_t)(t >> 32) - __umulhi(m, P);
     return min(r, r + P);
   }
-  // Montgomery product of two raw values (src/field/babybear.rs:133-137).
+  // Montgomery product without the final reduction: for a * b < p * 2^32 the
+  // sum t + m*p < 2^33 * p fits 64 bits and the result lies in [0, 2p).
+  __device__ static __forceinline__ uint32_t mul_lazy(uint32_t a, uint32_t b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * PINV_NEG;
+    return (uint32_t)((t + (uint64_t)m * P) >> 32);
+  }
+  // Montgomery product of two raw values (src/field/babybear.rs:133-137);
+  // canonical for a * b < p * 2^32 (a, b < p, or one of them below 2p).
   __device__ static __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
     uint64_t t = (uint64_t)a * b;
     uint32_t m = (uint32_t)t * PINV_NEG;

@@ -8,6 +8,12 @@ namespace sfn {
 constexpr int kPassThreads = 256;  // workgroup size of the pass kernels
 constexpr int kMaxLogTile = 11;    // largest LDS sub-transform (2^11 points)
 constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+// Powers of a Montgomery field's primitive 2^LOG_ROOT-th root g (LOG_ROOT <= 27),
+// Montgomery form: words [0, 2^14) hold g^j, words 2^14 + j hold g^(j * 2^14)
+// (j < 2^(LOG_ROOT - 14), the rest of the 2^13 words unused), so g^E for any
+// E < 2^LOG_ROOT is the product of two entries.
+constexpr uint32_t kPowLoBits = 14;
+constexpr size_t kPowWords = ((size_t)1 << kPowLoBits) + ((size_t)1 << 13);
 
 // Lane enumeration orders of a pass (a lane = one sub-transform instance).
 enum LaneOrder : uint32_t {
@@ -51,6 +57,9 @@ struct PassArgs {
   const uint32_t *pointf_m;       // LDE: per-block point factors sigma_b^(p << lo) (blocks x 2^LOG_T)
   uint32_t xcd_share_log;         // 2^s consecutive tiles share 128-byte lines: keep them on one XCD
   uint32_t tab_low_major;         // twist_m / post_m entries ordered (low, p) instead of (p, low)
+  // non-null (with twist_m null): the pass's four-step twist is generated
+  // from these root powers (kPowWords words) instead of read per point
+  const uint32_t *pow_m;
   // rows stored in bit-reversed order over this many bits (0: storage order);
   // tuned lane-contiguous passes with lo == 0 only
   uint32_t src_rev_bits, dst_rev_bits;

@@ -52,6 +52,7 @@ Context::~Context() {
   for (auto &b : retired) rel(b);
   rel(bb_small_pair);
   for (auto &b : mf_root) rel(b);
+  for (auto &b : mf_pow) rel(b);
   for (auto &kv : tables) {
     rel(kv.second.buf[0]);
     rel(kv.second.buf[1]);
@@ -324,6 +325,20 @@ sfn_status Context::root_table(int field, const uint2 **out) {
     if (s) return s;
   }
   *out = (const uint2 *)r.ptr;
+  return SFN_OK;
+}
+
+sfn_status Context::pow_table(int field, const uint32_t **out) {
+  std::lock_guard<std::recursive_mutex> g(mu);
+  DevBuf &r = mf_pow[mf_index(field)];
+  if (!r.ptr && NoAllocScope::current()) return not_resident(this, "the root power table");
+  if (!r.ptr) {
+    auto t = host::root_powers_m(host::monty_field(field), kPowLoBits, kPowWords);
+    if (t.empty()) return fail(SFN_ERR_INVALID_ARGUMENT, "root power table does not fit");
+    sfn_status s = upload_table(r, t.data(), t.size() * sizeof(uint32_t));
+    if (s) return s;
+  }
+  *out = (const uint32_t *)r.ptr;
   return SFN_OK;
 }
 
@@ -726,18 +741,7 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   a.batch_recip = batch_reciprocal(batch, a.instances);
   a.src_rev_bits = p.src_rev;
   a.dst_rev_bits = p.dst_rev;
-  if (field != SFN_FIELD_M31) {
-    const uint2 *f, *i;
-    sfn_status st = ctx->position_table(field, kMaxLogTile, &f, &i);
-    if (st) return st;
-    a.tw = dir == 0 ? f : i;
-    if (p.lo > 0) {
-      const uint2 *tw;
-      st = ctx->twist_table(field, p.lo, p.lo + p.log_t, dir == 1, &tw);
-      if (st) return st;
-      a.twist = tw;
-    }
-  } else {
+  if (field == SFN_FIELD_M31) {
     const uint32_t *t;
     sfn_status st = ctx->m31_table(p.log_n, dir == 1, &t);
     if (st) return st;
@@ -746,6 +750,24 @@ sfn_status make_pass_args(Context *ctx, int field, int dir, const PassDesc &p, c
   if (p.scale) {
     a.has_scale = 1;
     scale_factor(field, 1ull << log_n_real, &a.scale_w, &a.scale_q);
+  }
+  return SFN_OK;
+}
+
+// Generic-kernel tables of a Montgomery-field pass (after make_pass_args): the
+// position table and, for an outer pass, the four-step twist as Shoup pairs.
+// Only the generic kernels read them, so tuned passes never build them.
+static sfn_status fill_generic_args(Context *ctx, int field, int dir, const PassDesc &p, PassArgs *out) {
+  if (field == SFN_FIELD_M31) return SFN_OK;
+  PassArgs &a = *out;
+  const uint2 *f, *i;
+  sfn_status st = ctx->position_table(field, kMaxLogTile, &f, &i);
+  if (st) return st;
+  a.tw = dir == 0 ? f : i;
+  if (p.lo > 0) {
+    const uint2 *tw;
+    if ((st = ctx->twist_table(field, p.lo, p.lo + p.log_t, dir == 1, &tw))) return st;
+    a.twist = tw;
   }
   return SFN_OK;
 }
@@ -766,6 +788,15 @@ static bool fast_offsets_fit(const PassDesc &p, Layout ls, Layout ld) {
   return span(ls, p.src_rev) < 0xFFFFFFFFull && span(ld, p.dst_rev) < 0xFFFFFFFFull;
 }
 
+// SFNTT_TWIST_GEN=0 reads every four-step twist from its table (A/B measurement).
+static bool twist_gen_on() {
+  static const bool on = [] {
+    const char *e = getenv("SFNTT_TWIST_GEN");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // Tuned-kernel fields of a pass's argument block (after make_pass_args):
 // buffer extents, and for the Montgomery fields the root table and the
 // Montgomery-word twist.
@@ -781,9 +812,18 @@ static sfn_status fill_fast_args(Context *ctx, int field, int dir, const PassDes
   a.tw = root;
   a.twist = nullptr;
   if (p.lo > 0) {
-    const uint32_t *tm;
-    // lanes of one workgroup share their low row bits when columns run fastest
+    // lanes of one workgroup share their low row bits when columns run fastest:
+    // their twist reads are broadcasts of one table word.  Otherwise every
+    // point reads its own word, 4 bytes beside the 8 it moves, and the kernels
+    // generate the twist instead.
     a.tab_low_major = p.order == kColFastest;
+    if (!a.tab_low_major && twist_gen_on()) {
+      const uint32_t *pw;
+      if ((st = ctx->pow_table(field, &pw))) return st;
+      a.pow_m = pw;
+      return SFN_OK;
+    }
+    const uint32_t *tm;
     if ((st = ctx->twist_m_table(field, p.lo, p.lo + p.log_t, dir == 1, a.tab_low_major != 0, &tm))) return st;
     a.twist_m = tm;
   }
@@ -853,6 +893,7 @@ static sfn_status launch_desc(Context *ctx, int field, int dir, const PassDesc &
     choose_lanes(q, ls.cs == 1 && ls.rs != 1 ? SFN_LAYOUT_ROW_MAJOR : SFN_LAYOUT_SOA, batch, 4);
     return launch_desc(ctx, field, dir, q, src, dst, ls, ld, batch, log_n_real, s);
   }
+  if ((st0 = fill_generic_args(ctx, field, dir, p, &a))) return st0;
   (void)ctx->timing_mark(s, true, std::string("k_pass<") + field_tag(field) + (dir ? "inv," : "fwd,") +
                                      std::to_string(p.log_t) + "," + std::to_string(p.logr) + ">lo=" +
                                      std::to_string(p.lo));
@@ -1016,13 +1057,6 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     a.lane_contig = p.lane_contig;
     a.instances = (uint64_t)batch << lo;
     a.batch_recip = batch_reciprocal(batch, a.instances);
-    const uint2 *pair, *twi, *twf, *mid;
-    if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, true, &twi))) return st;
-    if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, false, &twf))) return st;
-    if ((st = bb_mid_table(log_n, added_bits, shift_std, &mid))) return st;
-    a.twist = twi;
-    a.twist_fwd = twf;
-    a.mid = mid;
     a.blocks = 1u << added_bits;
     a.dst_block_off = ((uint64_t)1 << log_n) * Lout.rs;
     // tuned middle kernel: lanes adjacent in memory, one item per thread
@@ -1062,6 +1096,14 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       if (e != hipSuccess) return hip_fail(e, "LDE middle pass launch");
       (void)timing_mark(s, false, "");
     } else {
+      // the generic middle kernel's tables: Shoup-pair twists and block factors
+      const uint2 *pair, *twi, *twf, *mid;
+      if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, true, &twi))) return st;
+      if ((st = twist_table(SFN_FIELD_BABYBEAR, lo, log_n, false, &twf))) return st;
+      if ((st = bb_mid_table(log_n, added_bits, shift_std, &mid))) return st;
+      a.twist = twi;
+      a.twist_fwd = twf;
+      a.mid = mid;
       if ((st = bb_small_pair_table(&pair))) return st;
       a.tw = pair;
       const uint64_t tiles = (a.instances + (1ull << lw) - 1) >> lw;
@@ -1205,6 +1247,7 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
         const size_t T = (size_t)1 << p.log_t;
         tile_bytes = std::max(tile_bytes, ((T + T / 32) << p.log_w) * 4);
       } else {
+        if ((st = fill_generic_args(ctx, m.field, m.dir, p, &d.a))) return st;
         L.lds = std::max(L.lds, pass_lds_bytes(m.field, p.log_t, p.log_w));
       }
       d.tile_begin = L.tiles;

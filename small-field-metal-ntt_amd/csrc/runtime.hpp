@@ -155,6 +155,7 @@ class Context {
   uint32_t mf_pos_log[2] = {0, 0};
   DevBuf bb_small_pair;    // [inv small | fwd small], 2^kMaxLogTile each (LDE middle pass)
   DevBuf mf_root[2];       // g_{2^11}^e, e < 2^11, Shoup pairs (tuned kernels)
+  DevBuf mf_pow[2];        // the field root's powers, Montgomery words (generated twists)
   // Per-shape tables (twists, LDE factors, M31 block twiddles): least recently
   // used entries are freed when the cache would exceed table_limit bytes (an
   // entry larger than the limit is still admitted alone).  Pinned entries are
@@ -215,6 +216,7 @@ class Context {
   sfn_status position_table(int field, uint32_t log_max, const uint2 **fwd, const uint2 **inv);
   sfn_status bb_small_pair_table(const uint2 **out);
   sfn_status root_table(int field, const uint2 **out);
+  sfn_status pow_table(int field, const uint32_t **out);
   // low_major: entry (low << (hi - lo)) | p instead of (p << lo) | low
   sfn_status twist_m_table(int field, uint32_t lo, uint32_t hi, bool inverse, bool low_major,
                            const uint32_t **out);

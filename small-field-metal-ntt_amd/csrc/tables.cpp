@@ -75,6 +75,20 @@ std::vector<Pair> root_table(const Monty &f, uint32_t log) {
   return t;
 }
 
+std::vector<uint32_t> root_powers_m(const Monty &f, uint32_t lo_bits, size_t words) {
+  const size_t n_lo = (size_t)1 << lo_bits, n_hi = (size_t)1 << (f.max_log - lo_bits);
+  std::vector<uint32_t> t(words, 0);
+  if (n_lo + n_hi > words) return {};
+  std::vector<uint32_t> p(n_lo);
+  fill_powers(p.data(), 0, n_lo, f.root_max, f.p);
+  for (size_t j = 0; j < n_lo; ++j) t[j] = f.to_monty(p[j]);
+  const uint32_t g_hi = powmod(f.root_max, n_lo, f.p);
+  p.assign(n_hi, 0);
+  fill_powers(p.data(), 0, n_hi, g_hi, f.p);
+  for (size_t j = 0; j < n_hi; ++j) t[n_lo + j] = f.to_monty(p[j]);
+  return t;
+}
+
 std::vector<Pair> twist_table(const Monty &f, uint32_t lo, uint32_t hi, bool inverse) {
   const uint32_t lt = hi - lo;
   const size_t size = (size_t)1 << hi;

@@ -4,6 +4,7 @@
 // chunks on up to 16 host threads; each chunk starts from a power (or circle
 // point) computed directly, so the result does not depend on the split.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>
@@ -79,6 +80,11 @@ std::vector<Pair> position_table(const Monty &f, uint32_t log_max, bool inverse)
 // Root table g_{2^log}^e, e < 2^log (Shoup pairs): one table for every layer
 // and both directions of a sub-transform of up to 2^log points.
 std::vector<Pair> root_table(const Monty &f, uint32_t log);
+
+// Montgomery-form powers of f's primitive 2^max_log-th root g in two levels:
+// [g^j, j < 2^lo_bits][g^(j << lo_bits), j < 2^(max_log - lo_bits)], the second
+// level padded with zeros to `words` in total.
+std::vector<uint32_t> root_powers_m(const Monty &f, uint32_t lo_bits, size_t words);
 
 // Four-step twist of layers [lo, hi): entry (p << lo) | low holds
 // g_hi^(+-low * bitrev(p, hi - lo)).
