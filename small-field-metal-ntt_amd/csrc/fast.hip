@@ -264,14 +264,14 @@ __device__ __forceinline__ void store_item(uint32_t (&v)[1 << K], const LaneInfo
       if (io.twist_gen) twist_gen<F, DIR, LOG_T, K, B0>(v, li, io, pbase);
       else mul_table<F, R>(v, io.twist, toff, 4u << (B0 + io.tps));
     }
-    if (io.has_post) {
-      if constexpr (LT) {
-        // keep the 32 table reads after the butterflies: hoisting them
-        // above the round costs 32 live registers at the peak
-        __builtin_amdgcn_sched_barrier(0);
-        mul_lds_table<F, R, B0>(v, io.lpost, pbase);
-      }
-      else mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
+    if constexpr (LT) {
+      // the LDE middle pass always has post factors (and leaves the values
+      // unreduced for them); keep the 32 table reads after the butterflies:
+      // hoisting them above the round costs 32 live registers at the peak
+      __builtin_amdgcn_sched_barrier(0);
+      mul_lds_table<F, R, B0>(v, io.lpost, pbase);
+    } else if (io.has_post) {
+      mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
     }
   }
   if (io.has_scale) {
@@ -329,7 +329,9 @@ __device__ __forceinline__ void run_round(uint32_t *tile, uint32_t log_w, const 
       for (int i = 0; i < R; ++i) v[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
     }
     if constexpr (!F::MONTY) tw.hb = li.hb;
-    round_regs<F, DIR, K, B0>(v, pbase, tw);
+    // the LDE middle pass's forward last round multiplies every value by its
+    // post factor (store_item), which takes unreduced values
+    round_regs<F, DIR, K, B0, LT && LAST && DIR == 0>(v, pbase, tw);
     if constexpr (LAST) {
       store_item<F, DIR, LOG_T, CONTIG, K, B0, LT>(v, li, io, pbase);
     } else {
@@ -633,7 +635,8 @@ __global__ __launch_bounds__(512, LW == 4 ? 4 : 3) void k_mid_fast(const PassArg
   uint32_t c[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) c[i] = tile[lidx<LOG_T, CONTIG>(lane, pbase | (i << B0), log_w)];
-  round_regs<F, 1, K, B0>(c, pbase, twi);
+  // unreduced: every coefficient is multiplied by a point factor next
+  round_regs<F, 1, K, B0, true>(c, pbase, twi);
   const uint64_t n = 1ull << a.log_n;
   io.has_twist = false;  // the forward twist is part of post_b
   io.has_post = true;

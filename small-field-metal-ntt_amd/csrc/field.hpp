@@ -147,6 +147,14 @@ __device__ __forceinline__ void bfly_ct_neg(uint32_t &x, uint32_t &y, typename F
   y = F::add(a, t);
 }
 
+// GS with both outputs unreduced, in [0, 2p): a + b and the lazy twiddle product.
+template <class F>
+__device__ __forceinline__ void bfly_gs_lazy(uint32_t &x, uint32_t &y, typename F::Tw w) {
+  const uint32_t a = x, b = y;
+  x = a + b;
+  y = F::mul_tw_lazy(a + F::P - b, w);
+}
+
 template <class F>
 __device__ __forceinline__ void bfly_gs(uint32_t &x, uint32_t &y, typename F::Tw w) {
   uint32_t a = x, b = y;
@@ -161,6 +169,30 @@ __device__ __forceinline__ void bfly_unit(uint32_t &x, uint32_t &y) {
   const uint32_t a = x, b = y;
   x = F::add(a, b);
   y = F::sub(a, b);
+}
+
+// Montgomery-field CT butterflies whose outputs may stay unreduced: with lx
+// (ly) the x (y) output is left in [0, 2p) as a + b or a - b + p (a, b < p),
+// for a value whose next use is the multiplied operand of a twiddle product
+// (Shoup's product takes any 32-bit value).  lx and ly are constants once the
+// caller's loops are unrolled.
+template <class F>
+__device__ __forceinline__ void bfly_unit_l(uint32_t &x, uint32_t &y, bool lx, bool ly) {
+  const uint32_t a = x, b = y;
+  x = lx ? a + b : F::add(a, b);
+  y = ly ? a + F::P - b : F::sub(a, b);
+}
+template <class F>
+__device__ __forceinline__ void bfly_ct_l(uint32_t &x, uint32_t &y, typename F::Tw w, bool lx, bool ly) {
+  const uint32_t t = F::mul_tw(y, w), a = x;
+  x = lx ? a + t : F::add(a, t);
+  y = ly ? a + F::P - t : F::sub(a, t);
+}
+template <class F>
+__device__ __forceinline__ void bfly_ct_neg_l(uint32_t &x, uint32_t &y, typename F::Tw wn, bool lx, bool ly) {
+  const uint32_t t = F::mul_tw(y, wn), a = x;
+  x = lx ? a + F::P - t : F::sub(a, t);
+  y = ly ? a + t : F::add(a, t);
 }
 
 }  // namespace sfn

@@ -126,8 +126,12 @@ struct TwCtx<M31, false> {
 // One register round: K layers on v[0..2^K) holding points pbase | (i << B0).
 // A fetcher with kNeg returns the negated twiddle -w for the inverse (CT)
 // butterflies, which then compute (x - (-w)y, x + (-w)y) = (x + wy, x - wy).
-template <class F, int DIR, int K, int B0, class TW>
+// LAZY_OUT (Montgomery fields): the round's last layer leaves its outputs
+// unreduced, in [0, 2p), for a caller that multiplies every output by a
+// Montgomery table factor next (mul_mont takes any 32-bit value).
+template <class F, int DIR, int K, int B0, bool LAZY_OUT = false, class TW>
 __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase, const TW &tw) {
+  static_assert(!LAZY_OUT || F::MONTY, "unreduced outputs: Montgomery fields only");
   constexpr int R = 1 << K;
   constexpr bool GS = F::MONTY == (DIR == 0);  // Montgomery fwd and M31 inv are GS
   if constexpr (DIR == 0) {
@@ -148,18 +152,30 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
           // BB: twiddle depends on i mod 2^j -> i = m + q*2^(j+1)
           // M31: twiddle depends on i >> (j+1) -> i = (m << (j+1)) + q
           const int i = F::MONTY ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          const bool lz = LAZY_OUT && j == 0;
           if constexpr (F::MONTY && B0 == 0) {
             if (m == 0) {  // position 0 of every layer: the twiddle is 1
-              bfly_unit<F>(v[i], v[i + (1 << j)]);
+              if constexpr (LAZY_OUT) bfly_unit_l<F>(v[i], v[i + (1 << j)], lz, lz);
+              else bfly_unit<F>(v[i], v[i + (1 << j)]);
               continue;
             }
           }
-          if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          if constexpr (GS && LAZY_OUT) {
+            if (lz) bfly_gs_lazy<F>(v[i], v[i + (1 << j)], w);
+            else bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          }
+          else if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
           else bfly_ct<F>(v[i], v[i + (1 << j)], w);
         }
       }
     }
   } else {
+    // Montgomery CT layers: both outputs of butterfly (i, i + 2^j) are the
+    // multiplied operands of layer j + 1 when bit j + 1 of i is set, and then
+    // stay unreduced (bfly_ct_l) unless that product is a twiddle-1 butterfly
+    // (position m = 0 in a round over the lowest point bits, only possible for
+    // the x output).  The round's last layer leaves canonical values.
+    constexpr bool kLazy = F::MONTY && !GS;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
 #pragma unroll
@@ -174,13 +190,19 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
 #pragma unroll
         for (int q = 0; q < (1 << (F::MONTY ? (K - 1 - j) : j)); ++q) {
           const int i = F::MONTY ? (m + (q << (j + 1))) : ((m << (j + 1)) + q);
+          const bool last = LAZY_OUT && j + 1 == K;
+          const bool ly = last || (kLazy && j + 1 < K && ((i >> (j + 1)) & 1) != 0);
+          const bool lx = last || (ly && !(B0 == 0 && m == 0));
           if constexpr (F::MONTY && B0 == 0) {
             if (m == 0) {
-              bfly_unit<F>(v[i], v[i + (1 << j)]);
+              if constexpr (kLazy) bfly_unit_l<F>(v[i], v[i + (1 << j)], lx, ly);
+              else bfly_unit<F>(v[i], v[i + (1 << j)]);
               continue;
             }
           }
           if constexpr (GS) bfly_gs<F>(v[i], v[i + (1 << j)], w);
+          else if constexpr (kLazy && TW::kNeg) bfly_ct_neg_l<F>(v[i], v[i + (1 << j)], w, lx, ly);
+          else if constexpr (kLazy) bfly_ct_l<F>(v[i], v[i + (1 << j)], w, lx, ly);
           else if constexpr (TW::kNeg) bfly_ct_neg<F>(v[i], v[i + (1 << j)], w);
           else bfly_ct<F>(v[i], v[i + (1 << j)], w);
         }
