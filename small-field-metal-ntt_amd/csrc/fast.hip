@@ -85,6 +85,7 @@ template <int LOG_T, int DIR>
 struct RootTw {
   static constexpr bool kNeg = false;
   static constexpr bool kFoldRoots = true;
+  static constexpr bool kSplitNeg = false;
   const uint2 *tab;
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
     uint32_t e = pos << (LOG_T - t - 1);
@@ -108,10 +109,22 @@ template <int LOG_T, int DIR>
 struct HalfRootTw {
   static constexpr bool kNeg = DIR == 1;
   static constexpr bool kFoldRoots = true;
+  static constexpr bool kSplitNeg = DIR == 1;
   const uint2 *tab;
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const {
     const uint32_t e = pos << (LOG_T - t - 1);
     return tab[DIR == 1 ? (1u << (LOG_T - 1)) - e : e];
+  }
+  // get(t, low | m << B0) of the inverse (entry T/2 - e) for m <= mmax,
+  // addressed from the layer's largest position: the per-thread base is
+  // shared by every m of the layer, and each m is a constant non-negative
+  // offset from it.  With low < 2^B0 and mmax < 2^(t - B0), the base index
+  // T/2 - (low + (mmax << B0)) << sh is at least T/2 - (2^t - 1) << sh = 2^sh.
+  template <int B0>
+  __device__ __forceinline__ uint2 get_split(int t, uint32_t low, uint32_t m, uint32_t mmax) const {
+    const uint32_t sh = LOG_T - t - 1;
+    const uint2 *base = tab + ((1u << (LOG_T - 1)) - (low << sh) - ((mmax << B0) << sh));
+    return base[((mmax - m) << B0) << sh];
   }
   template <class F>
   __device__ static __forceinline__ uint2 cget(int t, uint32_t pos) {

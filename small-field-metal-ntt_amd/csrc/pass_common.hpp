@@ -104,6 +104,7 @@ template <class F>
 struct TwCtx<F, true> {
   static constexpr bool kNeg = false;
   static constexpr bool kFoldRoots = false;
+  static constexpr bool kSplitNeg = false;
   const uint2 *stw;  // LDS
   __device__ __forceinline__ uint2 get(int t, uint32_t pos) const { return stw[(1u << t) - 1 + pos]; }
 };
@@ -112,6 +113,7 @@ template <>
 struct TwCtx<M31, false> {
   static constexpr bool kNeg = false;
   static constexpr bool kFoldRoots = false;
+  static constexpr bool kSplitNeg = false;
   const uint32_t *tbl;  // global, whole transform
   uint32_t n_log, lo, hi;
   uint64_t hb;
@@ -183,6 +185,8 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
         typename F::Tw w;
         if constexpr (F::MONTY) {
           if constexpr (B0 == 0 && TW::kFoldRoots) w = TW::template cget<F>(j, (uint32_t)m);
+          else if constexpr (TW::kSplitNeg)
+            w = tw.template get_split<B0>(B0 + j, pbase & ((1u << B0) - 1), (uint32_t)m, (1u << j) - 1);
           else w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
           w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
