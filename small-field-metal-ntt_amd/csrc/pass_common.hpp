@@ -123,6 +123,15 @@ struct TwCtx<M31, false> {
     const uint64_t off = (1ull << n_log) - (1ull << (n_log - l));
     return __ldg(tbl + off + ((hb << (hi - l - 1)) | blk));
   }
+  // get(t, base | m) for a base whose bits below m's range are clear: the
+  // words of one layer are then one per-thread address plus constant offsets m
+  // (folded into the loads), instead of an address computed per word.
+  __device__ __forceinline__ uint32_t get_split(int t, uint32_t base, uint32_t m) const {
+    const uint32_t l = lo + t;
+    const uint64_t off = (1ull << n_log) - (1ull << (n_log - l));
+    const uint32_t *p = tbl + off + (hb << (hi - l - 1)) + base;
+    return __ldg(p + m);
+  }
 };
 
 // One register round: K layers on v[0..2^K) holding points pbase | (i << B0).
@@ -147,7 +156,8 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
           if constexpr (B0 == 0 && TW::kFoldRoots) w = TW::template cget<F>(j, (uint32_t)m);
           else w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
-          w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
+          // pbase >> (B0 + j + 1) is zero below bit K - 1 - j, and m < 2^(K - 1 - j)
+          w = tw.get_split(B0 + j, pbase >> (B0 + j + 1), (uint32_t)m);
         }
 #pragma unroll
         for (int q = 0; q < (1 << (F::MONTY ? (K - 1 - j) : j)); ++q) {
@@ -189,7 +199,8 @@ __device__ __forceinline__ void round_regs(uint32_t (&v)[1 << K], uint32_t pbase
             w = tw.template get_split<B0>(B0 + j, pbase & ((1u << B0) - 1), (uint32_t)m, (1u << j) - 1);
           else w = tw.get(B0 + j, (pbase & ((1u << B0) - 1)) | ((uint32_t)m << B0));
         } else {
-          w = tw.get(B0 + j, (pbase >> (B0 + j + 1)) | (uint32_t)m);
+          // pbase >> (B0 + j + 1) is zero below bit K - 1 - j, and m < 2^(K - 1 - j)
+          w = tw.get_split(B0 + j, pbase >> (B0 + j + 1), (uint32_t)m);
         }
 #pragma unroll
         for (int q = 0; q < (1 << (F::MONTY ? (K - 1 - j) : j)); ++q) {
