@@ -37,38 +37,80 @@ def model_env():
 
 SUITES = ["ntt", "lde", "plans", "server", "faults", "cache", "noalloc",
           "schedules_full" if os.environ.get("SFNTT_MODEL_FULL") == "1" else "schedules"]
+FULL = os.environ.get("SFNTT_MODEL_FULL") == "1"
+
+# environment overrides of each test function's runs
+VARIANTS = {
+    "test_cpu_model_suite": {},
+    "test_cpu_model_reverse_thread_order": {"SFNTT_MODEL_ORDER": "reverse"},
+    "test_cpu_model_shuffled_thread_order": {"SFNTT_MODEL_ORDER": "shuffle:7"},
+    "test_cpu_model_few_compute_units": {"SFNTT_MODEL_CUS": "4"},
+}
 
 
-def _run_suite(env, suite):
-    r = subprocess.run([sys.executable, os.path.join(MODEL, "run_model.py"), suite], env=env,
-                       capture_output=True, text=True, timeout=1800)
-    tail = "\n".join((r.stdout + r.stderr).splitlines()[-40:])
-    assert r.returncode == 0, f"suite {suite} failed (rc {r.returncode}):\n{tail}"
-    assert f"suite {suite}: ok" in r.stdout, tail
+@pytest.fixture(scope="module")
+def model_runs(model_env, request, tmp_path_factory):
+    """Starts every model run the session selected from this module at once:
+    the runs are independent processes, so the module takes about as long as
+    its longest run instead of their sum.  Each test then waits for its own."""
+    logs = tmp_path_factory.mktemp("cpu_model")
+    runs = {}
+    for item in request.session.items:
+        name = getattr(item, "originalname", item.name)
+        if item.module.__name__ != __name__ or name not in VARIANTS or not hasattr(item, "callspec"):
+            continue
+        if item.get_closest_marker("skipif") and any(m.args and m.args[0] for m in item.iter_markers("skipif")):
+            continue
+        suite = item.callspec.params["suite"]
+        key = (name, suite)
+        if key in runs:
+            continue
+        log = logs / f"{name}_{suite}.log"
+        fh = open(log, "w")
+        proc = subprocess.Popen([sys.executable, os.path.join(MODEL, "run_model.py"), suite],
+                                env=dict(model_env, **VARIANTS[name]), stdout=fh, stderr=subprocess.STDOUT,
+                                text=True)
+        runs[key] = (proc, log, fh)
+    yield runs
+    for proc, _, fh in runs.values():
+        if proc.poll() is None:
+            proc.kill()
+        proc.wait()
+        fh.close()
+
+
+def _check(runs, name, suite):
+    proc, log, fh = runs[(name, suite)]
+    rc = proc.wait(timeout=1800)
+    fh.flush()
+    out = open(log).read()
+    tail = "\n".join(out.splitlines()[-40:])
+    assert rc == 0, f"suite {suite} failed (rc {rc}):\n{tail}"
+    assert f"suite {suite}: ok" in out, tail
 
 
 @pytest.mark.parametrize("suite", SUITES)
-def test_cpu_model_suite(model_env, suite):
-    _run_suite(model_env, suite)
+def test_cpu_model_suite(model_runs, suite):
+    _check(model_runs, "test_cpu_model_suite", suite)
 
 
 @pytest.mark.parametrize("suite", ["lde", "plans"])
-def test_cpu_model_reverse_thread_order(model_env, suite):
+def test_cpu_model_reverse_thread_order(model_runs, suite):
     """Workgroup threads run last to first between barriers: LDS data passed
     between threads without a barrier, or overwritten before another thread
     has read it, would give results that differ from the oracle.  With
     SFNTT_MODEL_FULL=1 every suite also runs in a shuffled order."""
-    _run_suite(dict(model_env, SFNTT_MODEL_ORDER="reverse"), suite)
+    _check(model_runs, "test_cpu_model_reverse_thread_order", suite)
 
 
-@pytest.mark.skipif(os.environ.get("SFNTT_MODEL_FULL") != "1", reason="full model sweep only")
+@pytest.mark.skipif(not FULL, reason="full model sweep only")
 @pytest.mark.parametrize("suite", ["ntt", "lde", "plans", "noalloc", "schedules"])
-def test_cpu_model_shuffled_thread_order(model_env, suite):
-    _run_suite(dict(model_env, SFNTT_MODEL_ORDER="shuffle:7"), suite)
+def test_cpu_model_shuffled_thread_order(model_runs, suite):
+    _check(model_runs, "test_cpu_model_shuffled_thread_order", suite)
 
 
 @pytest.mark.parametrize("suite", ["plans", "server"])
-def test_cpu_model_few_compute_units(model_env, suite):
+def test_cpu_model_few_compute_units(model_runs, suite):
     """Four compute units: the many-job launches, whose grid is sized to the
     device, run several tiles per workgroup."""
-    _run_suite(dict(model_env, SFNTT_MODEL_CUS="4"), suite)
+    _check(model_runs, "test_cpu_model_few_compute_units", suite)
