@@ -1189,13 +1189,12 @@ sfn_status build_multi(Context *ctx, const std::vector<MultiJob> &jobs_in, std::
     if (m.log_n == 0) continue;
     Schedule sc = ctx->schedule_for(m.field, m.log_n, m.batch, SFN_LAYOUT_SOA);
     // Many jobs fill the device together, so passes may be large: up to 2^11
-    // points, except that two 11-layer passes lose to three shorter ones on
-    // 2^22 columns (measured on 512 uneven columns: 2.50 vs 2.44 ms; the
-    // 2^11-point tile holds 8 lanes, so its strided pass moves 32-byte row
-    // pieces).  A tuned entry keeps its pass size; the radix is always the
-    // tuned kernel's.
-    if (!ctx->has_tuning(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA}))
-      sc.max_log_t = m.log_n > 2 * (kMaxLogTile - 1) + 1 ? kMaxLogTile - 1 : kMaxLogTile;
+    // points, also for 2^22 columns (two 11-layer passes against three
+    // shorter ones, measured on 512 uneven columns: 2.361-2.362 vs
+    // 2.373-2.384 ms; with the earlier 256-thread kernel the three passes won,
+    // 2.44 vs 2.50 ms).  A tuned entry keeps its pass size; the radix is
+    // always the tuned kernel's.
+    if (!ctx->has_tuning(TuneKey{m.field, m.log_n, m.batch, SFN_LAYOUT_SOA})) sc.max_log_t = kMaxLogTile;
     sc.logr = 5;
     jp[j] = plan_ntt_passes(m.field, SFN_VARIANT_AUTO, m.dir, m.log_n, m.batch, SFN_LAYOUT_SOA, sc, false);
     nl = std::max(nl, jp[j].size());
