@@ -1072,10 +1072,19 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
     if (allow_fast && (sc.logr == 5 || lo == 0) && choose_lanes_fast(q, layout, batch, sc.mid_log_w) &&
         (q.lane_contig || (lo == 0 && mid_aligned)) && fast_offsets_fit(q, mid_ls, mid_ld)) {
       const uint2 *root;
-      const uint32_t *tim, *post, *pointf;
+      const uint32_t *tim = nullptr, *post, *pointf;
       if ((st = root_table(SFN_FIELD_BABYBEAR, &root))) return st;
       const bool low_major = q.order == kColFastest;
-      if ((st = twist_m_table(SFN_FIELD_BABYBEAR, lo, log_n, true, low_major, &tim))) return st;
+      // lanes over low row bits (SoA) read a twist word per point: the kernel
+      // generates the inverse twist instead (as the tuned outer passes do)
+      const bool gen = q.lane_contig && !low_major && twist_gen_on();
+      if (gen) {
+        const uint32_t *pw;
+        if ((st = pow_table(SFN_FIELD_BABYBEAR, &pw))) return st;
+        a.pow_m = pw;
+      } else if ((st = twist_m_table(SFN_FIELD_BABYBEAR, lo, log_n, true, low_major, &tim))) {
+        return st;
+      }
       if ((st = bb_lde_fast(log_n, added_bits, shift_std, lo, low_major, &post, &pointf))) return st;
       a.tab_low_major = low_major;
       a.tw = root;
