@@ -152,7 +152,7 @@ __device__ __forceinline__ uint32_t lidx(uint32_t lane, uint32_t p, uint32_t log
 // Per-workgroup state shared by the rounds.  All offsets are relative to the
 // workgroup's first lane (lane 0 has the lowest address of the tile).
 struct RoundIO {
-  Rsrc src, dst, twist, pow;
+  Rsrc src, dst, twist, pow, lanef;
   const uint32_t *srcp;  // plain pointers (point-adjacent variant), already offset to lane 0
   uint32_t *dstp;
   uint64_t sbase0, dbase0;  // element index of lane 0's row 0 in src / dst
@@ -168,6 +168,7 @@ struct RoundIO {
   const uint32_t *ltw, *lpost;
   bool has_twist;
   bool twist_gen;  // has_twist: generate the twist from `pow` (else read `twist`)
+  bool post_gen;   // has_post: generate the post factors from `pow` and `lanef` (else read `post`)
   uint32_t has_scale, scale_w, scale_q;
   // LDE blocks: per-point factor table (one block) and post-twist table (one block)
   Rsrc pointf, post;
@@ -236,12 +237,15 @@ __device__ __forceinline__ uint32_t twist_root_pow(const RoundIO &io, uint32_t e
 // Montgomery product per point.  The walk stays in [0, 2p) (no final
 // reduction: s < 2p, r < p keep s * r < p * 2^32), and the product with the
 // value (< p) is reduced to [0, p).
-template <class F, int DIR, int LOG_T, int K, int B0>
+// SCALED: every factor also times the lane's word of `lanef` (the LDE's
+// post factors: forward twist x n^-1 sigma_b^low).
+template <class F, int DIR, int LOG_T, int K, int B0, bool SCALED = false>
 __device__ __forceinline__ void twist_gen(uint32_t (&v)[1 << K], const LaneInfo &li, const RoundIO &io,
                                           uint32_t pbase) {
   constexpr int R = 1 << K;
   const uint32_t brp = __brev(pbase) >> (32 - LOG_T);
   uint32_t s = twist_root_pow<F, DIR, LOG_T>(io, li.low * brp);
+  if constexpr (SCALED) s = F::mul(s, bload(io.lanef, li.low * 4u, 0));
   const uint32_t r = twist_root_pow<F, DIR, LOG_T>(io, li.low << (LOG_T - B0 - K));
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -284,7 +288,8 @@ __device__ __forceinline__ void store_item(uint32_t (&v)[1 << K], const LaneInfo
       __builtin_amdgcn_sched_barrier(0);
       mul_lds_table<F, R, B0>(v, io.lpost, pbase);
     } else if (io.has_post) {
-      mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
+      if (io.post_gen) twist_gen<F, DIR, LOG_T, K, B0, true>(v, li, io, pbase);
+      else mul_table<F, R>(v, io.post, toff, 4u << (B0 + io.tps));
     }
   }
   if (io.has_scale) {
@@ -437,6 +442,7 @@ __device__ __forceinline__ void setup(const PassArgs &a, LaneInfo *lanes, RoundI
   io.scale_w = a.scale_w;
   io.scale_q = a.scale_q;
   io.has_pointf = io.has_post = false;
+  io.post_gen = false;
 }
 
 // One tile (W lanes) of a tuned pass; any blockDim (threads loop over items).
@@ -676,7 +682,12 @@ __global__ __launch_bounds__(512, LW == 4 ? 4 : 3) void k_mid_fast(const PassArg
     } else {
       const Rsrc pf_r = make_rsrc(a.pointf_m + ((uint64_t)b << LOG_T), (uint64_t)4 << LOG_T);
       mul_table<F, R>(v, pf_r, pbase * (4u * R), 4u);
-      io.post = make_rsrc(a.post_m + b * n, n * 4);
+      if (a.lanef_m) {
+        io.lanef = make_rsrc(a.lanef_m + ((uint64_t)b << a.lo), (uint64_t)4 << a.lo);
+        io.post_gen = true;
+      } else {
+        io.post = make_rsrc(a.post_m + b * n, n * 4);
+      }
     }
     round_regs<F, 0, K, B0>(v, pbase, twf);
     __syncthreads();  // every thread is done reading the tile (and the tables in use)

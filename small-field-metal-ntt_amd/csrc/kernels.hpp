@@ -60,6 +60,9 @@ struct PassArgs {
   // non-null (with twist_m null): the pass's four-step twist is generated
   // from these root powers (kPowWords words) instead of read per point
   const uint32_t *pow_m;
+  // LDE middle pass over lanes of low row bits, with pow_m: the post factors
+  // are generated, times word b * 2^lo + low of this table (n^-1 sigma_b^low)
+  const uint32_t *lanef_m;
   // rows stored in bit-reversed order over this many bits (0: storage order);
   // tuned lane-contiguous passes with lo == 0 only
   uint32_t src_rev_bits, dst_rev_bits;

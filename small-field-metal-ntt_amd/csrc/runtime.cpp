@@ -378,6 +378,17 @@ sfn_status Context::bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t sh
       (const void **)post, (const void **)pointf);
 }
 
+sfn_status Context::bb_lde_lane(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
+                                const uint32_t **out) {
+  const TableKey k{kLdeLane, log_n, added_bits, shift_std, lo, 0};
+  return cached_table(
+      k,
+      [](const TableKey &k, std::vector<uint32_t> &h0, std::vector<uint32_t> &) {
+        h0 = host::bb_lde_lane_factors(k.a, k.b, k.c, k.d);
+      },
+      (const void **)out);
+}
+
 sfn_status Context::twist_table(int field, uint32_t lo, uint32_t hi, bool inverse, const uint2 **out) {
   const TableKey k{kTwistPair, (uint32_t)field, lo, hi, (uint32_t)inverse, 0};
   return cached_table(
@@ -1079,9 +1090,13 @@ sfn_status Context::run_lde(uint32_t log_n, uint32_t batch, uint32_t added_bits,
       // generates the inverse twist instead (as the tuned outer passes do)
       const bool gen = q.lane_contig && !low_major && twist_gen_on();
       if (gen) {
-        const uint32_t *pw;
+        // and its post factors: the forward twist walked as the inverse one,
+        // times the lane's n^-1 sigma_b^low
+        const uint32_t *pw, *lanef;
         if ((st = pow_table(SFN_FIELD_BABYBEAR, &pw))) return st;
+        if ((st = bb_lde_lane(log_n, added_bits, shift_std, lo, &lanef))) return st;
         a.pow_m = pw;
+        a.lanef_m = lanef;
       } else if ((st = twist_m_table(SFN_FIELD_BABYBEAR, lo, log_n, true, low_major, &tim))) {
         return st;
       }

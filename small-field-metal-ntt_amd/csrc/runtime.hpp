@@ -50,7 +50,8 @@ enum TableKind : uint32_t {
   kTwistMont = 2,   // four-step twist, Montgomery words: (field, lo, hi, inverse, low_major)
   kLdeMid = 3,      // generic LDE middle-pass blocks: (log_n, added_bits, shift)
   kLdeFast = 4,     // tuned LDE post / point-factor tables: (log_n, added_bits, shift, lo, low_major)
-  kM31Block = 5     // M31 block twiddles: (log_n, inverse)
+  kM31Block = 5,    // M31 block twiddles: (log_n, inverse)
+  kLdeLane = 6      // tuned LDE per-lane post factors n^-1 sigma_b^low: (log_n, added_bits, shift, lo)
 };
 struct TableKey {
   uint32_t kind, a, b, c, d, e;
@@ -222,6 +223,7 @@ class Context {
                            const uint32_t **out);
   sfn_status bb_lde_fast(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, bool low_major,
                          const uint32_t **post, const uint32_t **pointf);
+  sfn_status bb_lde_lane(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo, const uint32_t **out);
   sfn_status twist_table(int field, uint32_t lo, uint32_t hi, bool inverse, const uint2 **out);
   sfn_status bb_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, const uint2 **out);
   sfn_status m31_table(uint32_t log_n, bool inverse, const uint32_t **out);

@@ -185,6 +185,23 @@ void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std,
   }
 }
 
+std::vector<uint32_t> bb_lde_lane_factors(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo) {
+  const size_t n = (size_t)1 << log_n, L = (size_t)1 << lo;
+  const uint32_t blocks = 1u << added_bits;
+  const uint32_t inv_n = invmod((uint32_t)(n % kBBP), kBBP);
+  const uint32_t gN = bb_root(log_n + added_bits);
+  std::vector<uint32_t> t(L * blocks);
+  for (uint32_t blk = 0; blk < blocks; ++blk) {
+    const uint32_t sigma = mulmod(shift_std, powmod(gN, bitrev(blk, added_bits), kBBP), kBBP);
+    uint32_t w = inv_n;
+    for (size_t low = 0; low < L; ++low) {
+      t[blk * L + low] = bb_to_monty(w);
+      w = mulmod(w, sigma, kBBP);
+    }
+  }
+  return t;
+}
+
 namespace {
 struct Pt {
   uint32_t x, y;

@@ -105,6 +105,10 @@ std::vector<uint32_t> to_monty_words(const Monty &f, const std::vector<Pair> &t)
 void bb_lde_fast_tables(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo,
                         std::vector<uint32_t> &post, std::vector<uint32_t> &pointf);
 
+// The per-lane part of the post factors: block b entry low holds
+// n^-1 * sigma_b^low (Montgomery form), low < 2^lo, sigma_b = shift * g_{L+a}^bitrev(b, a).
+std::vector<uint32_t> bb_lde_lane_factors(uint32_t log_n, uint32_t added_bits, uint32_t shift_std, uint32_t lo);
+
 // Coset-LDE middle factors: block b (of 2^a) entry j holds
 // n^-1 * (shift * g_{L+a}^bitrev(b, a))^j, j < 2^L.
 std::vector<Pair> bb_lde_mid_table(uint32_t log_n, uint32_t added_bits, uint32_t shift_std);
